@@ -1,0 +1,189 @@
+"""Benchmark: GF(256) Good Cauchy block encode + erasure decode on one MI355X.
+
+Workload (one step): 4096 independent codewords, k=32, m=16, symbol_len=65536
+(8 GiB of data resident in HBM).  Encode all repairs, then decode every
+codeword from the k lowest surviving indices after erasing a seeded uniform
+subset of 1..16 of its 48 indices.  Throughput counts algorithmic bytes:
+(k+m)*L per encoded codeword plus 2*k*L per decoded codeword.
+
+Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+K, M, L, COUNT = 32, 16, 65536, 4096
+SEED_DATA = 0x5C25_0002
+SEED_ERASE = 0x5C25_E7A5
+HBM_PEAK_GBS = 8000.0
+
+
+def erasure_indices(count: int, k: int, m: int, max_r: int, seed: int) -> np.ndarray:
+    """Per codeword: erase a uniform subset of r ~ U{1..max_r} indices of n,
+    keep the k lowest survivors (ascending)."""
+    rng = np.random.default_rng(seed)
+    n = k + m
+    idx = np.empty((count, k), np.uint16)
+    for c in range(count):
+        r = int(rng.integers(1, max_r + 1))
+        erased = rng.permutation(n)[:r]
+        keep = np.ones(n, bool)
+        keep[erased] = False
+        idx[c] = np.flatnonzero(keep)[:k]
+    return idx
+
+
+def cpu_baseline(idx: np.ndarray, rep_gpu_sample: np.ndarray, target_s: float = 10.0) -> dict:
+    """The oracle's register-blocked GFNI port, timed on this host on a
+    bounded sample of the same workload; its outputs are checked against the
+    GPU's for the sampled codewords."""
+    import oracle
+
+    sample = 16
+    threads = max(1, min(16, os.cpu_count() or 1))
+    coeffs = oracle.good_cauchy_matrix(K, M)
+    data = oracle.splitmix_bytes_np(SEED_DATA, 0, sample * K * L)
+    t0 = time.perf_counter()
+    rep = oracle.gf256_encode_many(coeffs, K, M, L, data, sample, threads)
+    out = oracle.gf256_decode_many(coeffs, K, M, L, idx[:sample], data, rep, sample, threads)
+    one = time.perf_counter() - t0
+    if not np.array_equal(rep, rep_gpu_sample):
+        raise SystemExit("CPU and GPU repairs differ on the sampled codewords")
+    if not np.array_equal(out, data):
+        raise SystemExit("CPU decode did not round-trip")
+    reps = max(1, int(target_s / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rep = oracle.gf256_encode_many(coeffs, K, M, L, data, sample, threads)
+        oracle.gf256_decode_many(coeffs, K, M, L, idx[:sample], data, rep, sample, threads)
+    dt = time.perf_counter() - t0
+    nbytes = reps * sample * ((K + M) * L + 2 * K * L)
+    plan = {2: "GFNI+AVX2", 1: "AVX2 nibble", 0: "scalar"}[oracle.cpu_plan()]
+    return {
+        "value": round(nbytes / dt / 1e9, 3),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample} codewords x {reps} reps of the same encode+decode ({plan}, one codeword per thread)",
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--count", type=int, default=COUNT, help="codewords per step")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the workload shards by "
+                          "codeword with no exchange, so there is no multi-GPU step to time"}))
+        raise SystemExit(2)
+
+    import torch
+
+    from scrs_amd import Engine, Profile, batch_decoder, batch_encoder, fill_random_dev
+
+    torch.cuda.set_device(0)
+    count = args.count
+    prof = Profile.resolve(Engine.GOOD_CAUCHY, K, M, L)
+    enc, dec = batch_encoder(prof), batch_decoder(prof)
+    data = torch.empty(count * K * L, dtype=torch.uint8, device="cuda")
+    rep = torch.empty(count * M * L, dtype=torch.uint8, device="cuda")
+    out = torch.empty(count * K * L, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    fill_random_dev(data, data.numel(), SEED_DATA, 0, stream)
+    idx = erasure_indices(count, K, M, 16, SEED_ERASE)
+    torch.cuda.synchronize()
+
+    def step():
+        enc.encode_batch_dev(data, rep, count, stream)
+        dec.decode_batch_dev(idx, data, rep, out, count, stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record(stream)
+        enc.encode_batch_dev(data, rep, count, stream)
+        e1.record(stream)
+        dec.decode_batch_dev(idx, data, rep, out, count, stream)
+        e2.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    enc_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps
+    dec_ms = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps
+
+    # Bit-exactness: every codeword round-trips; sampled repairs vs the oracle.
+    if not torch.equal(out, data):
+        raise SystemExit("decode output differs from the original data")
+    enc_bytes = count * (K + M) * L
+    dec_bytes = count * 2 * K * L
+    ms_per_step = wall * 1e3 / args.steps
+    value = (enc_bytes + dec_bytes) / (wall / args.steps) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(idx, rep[: 16 * M * L].cpu().numpy())
+
+    enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
+    dec_gbs = dec_bytes / (dec_ms / 1e3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("gf256_encode_rows_terms_lds")
+    line = {
+        "metric": "encode/decode GB/s (bit-exact), % HBM roofline; batched vs plain GPU; p99 latency",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded splitmix64 bytes generated in HBM; seeded erasure subsets)",
+        "config": {
+            "workload": "GF(256) Good Cauchy block encode + batch decode, k=32 m=16 symbol_len=65536, "
+                        "4096 codewords (8 GiB data), erase U{1..16} of 48 per codeword",
+            "codewords": count, "k": K, "m": M, "symbol_len": L,
+            "kernel": "lds-nibble-tables",
+        },
+        "encode_gbs": round(enc_gbs, 2),
+        "decode_gbs": round(dec_gbs, 2),
+        "encode_ms": round(enc_ms, 4),
+        "decode_ms": round(dec_ms, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "rows_terms_lds_kernel (encode launch)",
+            "achieved": round(enc_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(enc_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

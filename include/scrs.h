@@ -1,0 +1,163 @@
+/*
+ * scrs.h - C ABI of the SCRS payload-arithmetic engine for AMD MI355X (gfx950).
+ *
+ * The reference exposes its codecs as Rust traits.  Each entry point below is
+ * the C-ABI form of one trait method or constructor; the comment names the
+ * reference item it replaces (paths relative to the reference repository).
+ * Handles are opaque.  Every call returns an scrs_status whose code and
+ * fields mirror the reference's error variants (scrs_status.h), and the
+ * order of checks matches the reference, so the same bad input reports the
+ * same error.
+ *
+ * Buffers:
+ *   - Host-slice calls (`*_into`, `*_into_with`) take ordinary host memory,
+ *     as the reference's `&[u8]` / `&mut [u8]` arguments do; they stage the
+ *     bytes through device memory and return when the result is in host
+ *     memory.
+ *   - Device calls (`*_dev`) take device pointers and a hipStream_t (passed as
+ *     void*, null = the null stream) and return once the work is enqueued.
+ *
+ * Threading: encoders may be used from several threads at once (the
+ * reference's encoders take &self); decoders and scratches need external
+ * synchronisation (the reference's decoders take &mut self).
+ */
+#ifndef SCRS_H
+#define SCRS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "scrs_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Validated geometry (codec.rs:54-103). */
+typedef struct scrs_profile {
+    int32_t engine; /* scrs_engine */
+    uint64_t k;
+    uint64_t m;
+    uint64_t symbol_len;
+} scrs_profile;
+
+typedef struct scrs_batch_encoder scrs_batch_encoder;
+typedef struct scrs_batch_decoder scrs_batch_decoder;
+typedef struct scrs_encode_scratch scrs_encode_scratch;
+typedef struct scrs_decode_scratch scrs_decode_scratch;
+
+/* ---- Library ---------------------------------------------------------- */
+
+/* Version of this ABI (no reference counterpart). */
+uint32_t scrs_abi_version(void);
+
+/* Human-readable name of a status code (no reference counterpart). */
+const char *scrs_status_name(int32_t code);
+
+/* ---- Profile selection ------------------------------------------------ */
+
+/* Engine::field (codec.rs:42-50).  Returns -1 for an unknown engine. */
+int32_t scrs_engine_field(int32_t engine);
+
+/* recommended_gf16_engine (lib.rs:85-93): AFFT when k+m > 256 or 3m >= k,
+ * else Tower.  Returns an scrs_engine. */
+int32_t scrs_recommended_gf16_engine(uint64_t k, uint64_t m);
+
+/* Profile::resolve (selector.rs:40-61).  Check order: ZeroDimension ->
+ * ZeroSymbolLen -> OddSymbolLen -> TooManySymbols{cap}. */
+scrs_status scrs_profile_resolve(int32_t engine, uint64_t k, uint64_t m, uint64_t symbol_len,
+                                 scrs_profile *out);
+
+/* Profile::recommended (selector.rs:67-87): GF(256) picks Good Cauchy when
+ * k+m <= 255, else Standard; GF(65536) follows scrs_recommended_gf16_engine. */
+scrs_status scrs_profile_recommended(int32_t field, uint64_t k, uint64_t m, uint64_t symbol_len,
+                                     scrs_profile *out);
+
+/* ---- BatchEncoder (codec.rs:136-151) ------------------------------------ */
+
+/* batch_encoder(&profile) (selector.rs:457-467).  Tower is incremental-only:
+ * UnsupportedMode{engine}. */
+scrs_status scrs_batch_encoder_create(const scrs_profile *profile, scrs_batch_encoder **out);
+void scrs_batch_encoder_destroy(scrs_batch_encoder *enc);
+/* Coded::{k, m, symbol_len} (codec.rs:106-117). */
+scrs_status scrs_batch_encoder_profile(const scrs_batch_encoder *enc, scrs_profile *out);
+
+/* BatchEncoder::scratch (codec.rs:140). */
+scrs_status scrs_encode_scratch_create(const scrs_batch_encoder *enc, scrs_encode_scratch **out);
+void scrs_encode_scratch_destroy(scrs_encode_scratch *scratch);
+
+/* BatchEncoder::encode_into (codec.rs:143; batch/codec.rs:156-193).
+ * data = k * symbol_len bytes, repairs = m * symbol_len bytes, overwritten.
+ * Check order: WrongInputLen -> WrongOutputLen. */
+scrs_status scrs_encode_into(const scrs_batch_encoder *enc, const uint8_t *data, uint64_t data_len,
+                             uint8_t *repairs, uint64_t repairs_len);
+
+/* BatchEncoder::encode_into_with (codec.rs:145-150; selector.rs:510-528).
+ * A scratch of the wrong engine kind: ScratchMismatch (checked first). */
+scrs_status scrs_encode_into_with(const scrs_batch_encoder *enc, const uint8_t *data,
+                                  uint64_t data_len, uint8_t *repairs, uint64_t repairs_len,
+                                  scrs_encode_scratch *scratch);
+
+/* Batched device encode (no reference counterpart; the batch form of
+ * encode_into): `count` codewords back to back, codeword c's data at
+ * data_dev + c*k*L and its repairs written to repairs_dev + c*m*L. */
+scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, const uint8_t *data_dev,
+                                  uint8_t *repairs_dev, uint64_t count, void *stream);
+
+/* ---- BatchDecoder (codec.rs:157-175) ------------------------------------ */
+
+/* batch_decoder(&profile) (selector.rs:270-279). */
+scrs_status scrs_batch_decoder_create(const scrs_profile *profile, scrs_batch_decoder **out);
+void scrs_batch_decoder_destroy(scrs_batch_decoder *dec);
+scrs_status scrs_batch_decoder_profile(const scrs_batch_decoder *dec, scrs_profile *out);
+
+/* BatchDecoder::scratch (codec.rs:161; batch/codec.rs:245-258). */
+scrs_status scrs_decode_scratch_create(const scrs_batch_decoder *dec, scrs_decode_scratch **out);
+void scrs_decode_scratch_destroy(scrs_decode_scratch *scratch);
+
+/* BatchDecoder::decode_into (codec.rs:163-167; batch/codec.rs:265-271).
+ * `count` symbols: indices[i] in 0..n with payloads[i] of payload_lens[i]
+ * bytes.  out = k * symbol_len bytes, every byte overwritten.
+ * GF(256) check order: WrongOutputLen -> WrongCount -> per symbol
+ * IndexOutOfRange -> DuplicateIndex -> WrongPayloadLen
+ * (batch/codec.rs:293-331). */
+scrs_status scrs_decode_into(scrs_batch_decoder *dec, const uint64_t *indices,
+                             const uint8_t *const *payloads, const uint64_t *payload_lens,
+                             uint64_t count, uint8_t *out, uint64_t out_len);
+
+/* BatchDecoder::decode_into_with (codec.rs:169-174; batch/codec.rs:278-406).
+ * A scratch from another geometry: ScratchMismatch (checked first). */
+scrs_status scrs_decode_into_with(scrs_batch_decoder *dec, const uint64_t *indices,
+                                  const uint8_t *const *payloads, const uint64_t *payload_lens,
+                                  uint64_t count, uint8_t *out, uint64_t out_len,
+                                  scrs_decode_scratch *scratch);
+
+/* Batched device decode (no reference counterpart; the batch form of
+ * decode_into).  For codeword c, indices[c*k .. c*k+k) (host memory) name
+ * the k received symbols; symbol i < k is read from data_dev + (c*k+i)*L and
+ * symbol k+j from repairs_dev + (c*m+j)*L.  The k data symbols are written
+ * to out_dev + c*k*L.  Indices are validated on the host with the
+ * decode_into check order before anything is enqueued. */
+scrs_status scrs_decode_batch_dev(scrs_batch_decoder *dec, const uint16_t *indices,
+                                  const uint8_t *data_dev, const uint8_t *repairs_dev,
+                                  uint8_t *out_dev, uint64_t count, void *stream);
+
+/* Same, with the received payloads given as a device array of `count * k`
+ * device pointers (payload_ptrs_dev[c*k + t] is the payload of indices[c*k+t]). */
+scrs_status scrs_decode_batch_ptrs_dev(scrs_batch_decoder *dec, const uint16_t *indices,
+                                       const uint8_t *const *payload_ptrs_dev, uint8_t *out_dev,
+                                       uint64_t count, void *stream);
+
+/* ---- Utilities (no reference counterpart) ----------------------------- */
+
+/* Fill len bytes of device memory with the seeded splitmix64 counter stream
+ * (byte i = byte i%8 of splitmix64 word (byte_offset+i)/8 of state seed);
+ * byte_offset must be a multiple of 8. */
+scrs_status scrs_fill_random_dev(uint8_t *dst_dev, uint64_t len, uint64_t seed,
+                                 uint64_t byte_offset, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCRS_H */

@@ -1,0 +1,22 @@
+"""SCRS payload arithmetic on AMD MI355X (gfx950).
+
+The codecs live in the HIP library ``libscrs_hip.so`` behind the C ABI of
+``include/scrs.h``; this package is its Python binding, mirroring the
+reference's ``Profile`` selection and codec traits.
+"""
+from .codec import (  # noqa: F401
+    BatchDecoder,
+    BatchEncoder,
+    ConfigError,
+    DecodeError,
+    DeviceError,
+    EncodeError,
+    Engine,
+    Field,
+    Profile,
+    ScrsError,
+    batch_decoder,
+    batch_encoder,
+    fill_random_dev,
+    recommended_gf16_engine,
+)

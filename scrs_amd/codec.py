@@ -1,0 +1,387 @@
+"""Python mirror of the reference's codec surface over the HIP C ABI.
+
+Names, argument meanings and errors follow the reference crate:
+``Profile::resolve`` / ``Profile::recommended`` (scrs/src/selector.rs:40,67),
+the ``BatchEncoder`` / ``BatchDecoder`` / ``Decoder`` / ``IncrementalEncoder``
+traits (scrs/src/codec.rs:122-201) and the error enums
+(scrs/src/error.rs:12-126).  Every method calls into ``libscrs_hip.so``;
+nothing here computes payload bytes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from enum import IntEnum
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib
+
+
+class Field(IntEnum):
+    """codec.rs:20-26."""
+
+    GF256 = 0
+    GF65536 = 1
+
+
+class Engine(IntEnum):
+    """codec.rs:30-40."""
+
+    STANDARD_CAUCHY = 0
+    GOOD_CAUCHY = 1
+    TOWER = 2
+    AFFT = 3
+
+    def field(self) -> Field:
+        return Field(_lib.load().scrs_engine_field(int(self)))
+
+
+# ---------------------------------------------------------------------------
+# Errors
+# ---------------------------------------------------------------------------
+
+
+class ScrsError(Exception):
+    """Base of every error raised by the codec surface."""
+
+    def __init__(self, variant: str, **fields):
+        self.variant = variant
+        self.fields = fields
+        body = ", ".join(f"{k}: {v}" for k, v in fields.items())
+        super().__init__(f"{type(self).__name__}::{variant}" + (f" {{ {body} }}" if body else ""))
+
+    def __eq__(self, other):  # compare like the reference's PartialEq errors
+        return type(self) is type(other) and self.variant == other.variant and self.fields == other.fields
+
+    def __hash__(self):
+        return hash((type(self).__name__, self.variant, tuple(sorted(self.fields.items()))))
+
+
+class ConfigError(ScrsError):
+    """error.rs:12."""
+
+
+class EncodeError(ScrsError):
+    """error.rs:35."""
+
+
+class DecodeError(ScrsError):
+    """error.rs:75."""
+
+
+class DeviceError(ScrsError):
+    """A HIP runtime failure or C-ABI misuse (no reference counterpart)."""
+
+
+# code -> (class, variant, field names for a and b)
+_CODES = {
+    10: (ConfigError, "ZeroDimension", ()),
+    11: (ConfigError, "TooManySymbols", ("cap",)),
+    12: (ConfigError, "ZeroSymbolLen", ()),
+    13: (ConfigError, "OddSymbolLen", ()),
+    14: (ConfigError, "UnsupportedMode", ("engine",)),
+    20: (EncodeError, "WrongInputLen", ("expected", "got")),
+    21: (EncodeError, "IndexOutOfRange", ("index", "n")),
+    22: (EncodeError, "WrongPayloadLen", ("expected", "got")),
+    23: (EncodeError, "DuplicateData", ("index",)),
+    24: (EncodeError, "WrongOutputLen", ("expected", "got")),
+    25: (EncodeError, "ScratchMismatch", ()),
+    30: (DecodeError, "WrongCount", ("expected", "got")),
+    31: (DecodeError, "IndexOutOfRange", ("index", "n")),
+    32: (DecodeError, "DuplicateIndex", ("index",)),
+    33: (DecodeError, "WrongPayloadLen", ("expected", "got")),
+    34: (DecodeError, "TooManySymbols", ("cap", "received")),
+    35: (DecodeError, "InsufficientRank", ("rank", "k")),
+    36: (DecodeError, "WrongOutputLen", ("expected", "got")),
+    37: (DecodeError, "ScratchMismatch", ()),
+    90: (DeviceError, "InvalidArgument", ("position",)),
+    91: (DeviceError, "Device", ("hip_error",)),
+    92: (DeviceError, "OutOfMemory", ("bytes",)),
+    93: (DeviceError, "CapacityExceeded", ("limit",)),
+}
+
+
+def error_from_status(code: int, a: int, b: int) -> ScrsError:
+    cls, variant, names = _CODES.get(code, (DeviceError, f"Unknown{code}", ()))
+    vals = (a, b)
+    fields = {name: vals[i] for i, name in enumerate(names)}
+    if cls is ConfigError and variant == "UnsupportedMode":
+        fields["engine"] = Engine(a)
+    return cls(variant, **fields)
+
+
+def check(status: _lib.Status) -> None:
+    if status.code != 0:
+        raise error_from_status(status.code, status.a, status.b)
+
+
+# ---------------------------------------------------------------------------
+# Buffers
+# ---------------------------------------------------------------------------
+
+
+def _in_buf(buf) -> tuple[int, int, object]:
+    """(address, length, keepalive) of a readable host byte buffer."""
+    if isinstance(buf, np.ndarray):
+        arr = np.ascontiguousarray(buf).view(np.uint8).reshape(-1)
+    else:
+        arr = np.frombuffer(memoryview(buf).cast("B"), dtype=np.uint8) if len(memoryview(buf)) else np.zeros(0, np.uint8)
+    if arr.size == 0:
+        return 0, 0, arr
+    return arr.ctypes.data, arr.size, arr
+
+
+def _out_buf(buf) -> tuple[int, int, object]:
+    """(address, length, keepalive) of a writable host byte buffer."""
+    if isinstance(buf, np.ndarray):
+        if not buf.flags.c_contiguous or not buf.flags.writeable:
+            raise ValueError("output array must be C-contiguous and writable")
+        arr = buf.view(np.uint8).reshape(-1)
+    else:
+        mv = memoryview(buf)
+        if mv.readonly:
+            raise ValueError("output buffer must be writable")
+        arr = np.frombuffer(mv.cast("B"), dtype=np.uint8) if len(mv) else np.zeros(0, np.uint8)
+    if arr.size == 0:
+        return 0, 0, arr
+    return arr.ctypes.data, arr.size, arr
+
+
+def _dev_ptr(t) -> int:
+    """Device address of a torch tensor (or a raw integer address)."""
+    return t if isinstance(t, int) else int(t.data_ptr())
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+# ---------------------------------------------------------------------------
+# Profile
+# ---------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class Profile:
+    """A validated geometry (codec.rs:54-103)."""
+
+    engine: Engine
+    k: int
+    m: int
+    symbol_len: int
+
+    @property
+    def n(self) -> int:
+        return self.k + self.m
+
+    @property
+    def field(self) -> Field:
+        return self.engine.field()
+
+    def _struct(self) -> _lib.ProfileStruct:
+        return _lib.ProfileStruct(int(self.engine), self.k, self.m, self.symbol_len)
+
+    @staticmethod
+    def resolve(engine: Engine, k: int, m: int, symbol_len: int) -> "Profile":
+        """Profile::resolve (selector.rs:40-61)."""
+        out = _lib.ProfileStruct()
+        check(_lib.load().scrs_profile_resolve(int(engine), k, m, symbol_len, C.byref(out)))
+        return Profile(Engine(out.engine), out.k, out.m, out.symbol_len)
+
+    @staticmethod
+    def recommended(field: Field, k: int, m: int, symbol_len: int) -> "Profile":
+        """Profile::recommended (selector.rs:67-87)."""
+        out = _lib.ProfileStruct()
+        check(_lib.load().scrs_profile_recommended(int(field), k, m, symbol_len, C.byref(out)))
+        return Profile(Engine(out.engine), out.k, out.m, out.symbol_len)
+
+
+def recommended_gf16_engine(k: int, m: int) -> Engine:
+    """lib.rs:85-93."""
+    return Engine(_lib.load().scrs_recommended_gf16_engine(k, m))
+
+
+class _Handle:
+    _destroy_name = ""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            getattr(_lib.load(), self._destroy_name)(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class EncodeScratch(_Handle):
+    _destroy_name = "scrs_encode_scratch_destroy"
+
+
+class DecodeScratch(_Handle):
+    _destroy_name = "scrs_decode_scratch_destroy"
+
+
+class _Coded(_Handle):
+    profile: Profile
+
+    @property
+    def k(self) -> int:
+        return self.profile.k
+
+    @property
+    def m(self) -> int:
+        return self.profile.m
+
+    @property
+    def n(self) -> int:
+        return self.profile.n
+
+    @property
+    def symbol_len(self) -> int:
+        return self.profile.symbol_len
+
+
+# ---------------------------------------------------------------------------
+# BatchEncoder
+# ---------------------------------------------------------------------------
+
+
+class BatchEncoder(_Coded):
+    """BatchEncoder (codec.rs:136-151) on the device."""
+
+    _destroy_name = "scrs_batch_encoder_destroy"
+
+    def __init__(self, profile: Profile):
+        h = C.c_void_p()
+        check(_lib.load().scrs_batch_encoder_create(C.byref(profile._struct()), C.byref(h)))
+        super().__init__(h.value)
+        self.profile = profile
+
+    def scratch(self) -> EncodeScratch:
+        h = C.c_void_p()
+        check(_lib.load().scrs_encode_scratch_create(self._h, C.byref(h)))
+        return EncodeScratch(h.value)
+
+    def encode_into(self, data, repairs) -> None:
+        dp, dn, _k1 = _in_buf(data)
+        rp, rn, _k2 = _out_buf(repairs)
+        check(_lib.load().scrs_encode_into(self._h, dp, dn, rp, rn))
+
+    def encode_into_with(self, data, repairs, scratch: EncodeScratch) -> None:
+        dp, dn, _k1 = _in_buf(data)
+        rp, rn, _k2 = _out_buf(repairs)
+        check(_lib.load().scrs_encode_into_with(self._h, dp, dn, rp, rn, scratch._h))
+
+    def encode(self, data) -> np.ndarray:
+        """Convenience: return the m*symbol_len repair bytes."""
+        out = np.empty(self.m * self.symbol_len, np.uint8)
+        self.encode_into(data, out)
+        return out
+
+    def encode_batch_dev(self, data_dev, repairs_dev, count: int, stream=None) -> None:
+        check(_lib.load().scrs_encode_batch_dev(self._h, _dev_ptr(data_dev), _dev_ptr(repairs_dev), count,
+                                                _stream_ptr(stream)))
+
+
+def batch_encoder(profile: Profile) -> BatchEncoder:
+    """batch_encoder (selector.rs:457-467)."""
+    return BatchEncoder(profile)
+
+
+# ---------------------------------------------------------------------------
+# BatchDecoder
+# ---------------------------------------------------------------------------
+
+
+def _symbols_arrays(symbols: Sequence[tuple[int, object]]):
+    n = len(symbols)
+    idx = (C.c_uint64 * max(n, 1))()
+    ptrs = (C.c_void_p * max(n, 1))()
+    lens = (C.c_uint64 * max(n, 1))()
+    keep = []
+    for i, (index, payload) in enumerate(symbols):
+        p, ln, k = _in_buf(payload)
+        keep.append(k)
+        idx[i] = index
+        # A zero-length payload still needs a non-null address.
+        ptrs[i] = p if p else C.addressof(lens)
+        lens[i] = ln
+    return idx, ptrs, lens, keep
+
+
+class BatchDecoder(_Coded):
+    """BatchDecoder (codec.rs:157-175) on the device."""
+
+    _destroy_name = "scrs_batch_decoder_destroy"
+
+    def __init__(self, profile: Profile):
+        h = C.c_void_p()
+        check(_lib.load().scrs_batch_decoder_create(C.byref(profile._struct()), C.byref(h)))
+        super().__init__(h.value)
+        self.profile = profile
+
+    def scratch(self) -> DecodeScratch:
+        h = C.c_void_p()
+        check(_lib.load().scrs_decode_scratch_create(self._h, C.byref(h)))
+        return DecodeScratch(h.value)
+
+    def decode_into(self, symbols: Sequence[tuple[int, object]], out) -> None:
+        idx, ptrs, lens, _keep = _symbols_arrays(symbols)
+        op, on, _k = _out_buf(out)
+        check(_lib.load().scrs_decode_into(self._h, idx, ptrs, lens, len(symbols), op, on))
+
+    def decode_into_with(self, symbols: Sequence[tuple[int, object]], out, scratch: DecodeScratch) -> None:
+        idx, ptrs, lens, _keep = _symbols_arrays(symbols)
+        op, on, _k = _out_buf(out)
+        check(_lib.load().scrs_decode_into_with(self._h, idx, ptrs, lens, len(symbols), op, on, scratch._h))
+
+    def decode(self, symbols: Sequence[tuple[int, object]]) -> np.ndarray:
+        out = np.empty(self.k * self.symbol_len, np.uint8)
+        self.decode_into(symbols, out)
+        return out
+
+    def decode_batch_dev(self, indices: np.ndarray, data_dev, repairs_dev, out_dev, count: int,
+                         stream=None) -> None:
+        idx = np.ascontiguousarray(indices, dtype=np.uint16)
+        if idx.size != count * self.k:
+            raise ValueError("indices must hold count * k entries")
+        check(_lib.load().scrs_decode_batch_dev(self._h, idx.ctypes.data_as(_lib.u16p), _dev_ptr(data_dev),
+                                                _dev_ptr(repairs_dev), _dev_ptr(out_dev), count,
+                                                _stream_ptr(stream)))
+
+    def decode_batch_ptrs_dev(self, indices: np.ndarray, payload_ptrs_dev, out_dev, count: int,
+                              stream=None) -> None:
+        idx = np.ascontiguousarray(indices, dtype=np.uint16)
+        if idx.size != count * self.k:
+            raise ValueError("indices must hold count * k entries")
+        check(_lib.load().scrs_decode_batch_ptrs_dev(self._h, idx.ctypes.data_as(_lib.u16p),
+                                                     _dev_ptr(payload_ptrs_dev), _dev_ptr(out_dev), count,
+                                                     _stream_ptr(stream)))
+
+
+def batch_decoder(profile: Profile) -> BatchDecoder:
+    """batch_decoder (selector.rs:270-279)."""
+    return BatchDecoder(profile)
+
+
+def fill_random_dev(dst_dev, length: int, seed: int, byte_offset: int = 0, stream=None) -> None:
+    """Seeded splitmix64 counter-stream bytes written on the device."""
+    check(_lib.load().scrs_fill_random_dev(_dev_ptr(dst_dev), length, seed, byte_offset, _stream_ptr(stream)))

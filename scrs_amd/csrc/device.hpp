@@ -1,0 +1,80 @@
+// device.hpp - small RAII helpers over the HIP runtime used by the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <utility>
+
+namespace scrs {
+
+// Device allocation that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t size = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        size = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= size) return hipSuccess;
+        release();
+        hipError_t err = hipMalloc(&ptr, n);
+        if (err != hipSuccess) {
+            ptr = nullptr;
+            return err;
+        }
+        size = n;
+        return hipSuccess;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Pinned host allocation that only grows.
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t size = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        size = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= size) return hipSuccess;
+        release();
+        hipError_t err = hipHostMalloc(&ptr, n, hipHostMallocDefault);
+        if (err != hipSuccess) {
+            ptr = nullptr;
+            return err;
+        }
+        size = n;
+        return hipSuccess;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Makes `device` current for the scope and restores the caller's device.
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    explicit DeviceGuard(int device) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != device) {
+            changed = hipSetDevice(device) == hipSuccess;
+        }
+    }
+    ~DeviceGuard() {
+        if (changed) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace scrs

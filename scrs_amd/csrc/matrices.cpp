@@ -1,0 +1,85 @@
+// matrices.cpp - coefficient matrices and packed product tables.
+#include "matrices.hpp"
+
+namespace scrs {
+
+const Gf256 &gf256() {
+    static const Gf256 field;
+    return field;
+}
+
+std::vector<uint8_t> standard_cauchy_matrix(size_t k, size_t m) {
+    const Gf256 &f = gf256();
+    std::vector<uint8_t> c(k * m);
+    for (size_t i = 0; i < k; i++)
+        for (size_t j = 0; j < m; j++)
+            c[i * m + j] = f.inv(static_cast<uint8_t>(i ^ (k + j)));
+    return c;
+}
+
+std::vector<uint8_t> good_cauchy_matrix(size_t k, size_t m) {
+    const Gf256 &f = gf256();
+    std::vector<uint8_t> base(k + m, 0);
+    for (size_t d = 1; d < k + m; d++) base[d] = f.inv(static_cast<uint8_t>(1 ^ f.pow3(d)));
+    std::vector<uint8_t> c(k * m);
+    for (size_t i = 0; i < k; i++) {
+        uint8_t scale = f.inv(f.pow3(i));
+        for (size_t j = 0; j < m; j++) c[i * m + j] = f.m(scale, base[k + j - i]);
+    }
+    return c;
+}
+
+std::vector<uint16_t> tower_cauchy_matrix(size_t k, size_t m) {
+    const size_t n = k + m;
+    std::vector<uint16_t> powers(n);
+    uint16_t v = 1;
+    for (size_t i = 0; i < n; i++) {
+        powers[i] = v;
+        v = Gf16::mul(v, Gf16::kGenerator);
+    }
+    // base[d] = 1 / (1 + g^d) for d = 1..n-1, inverted together with one
+    // field inversion (prefix products, then one reciprocal walked back).
+    std::vector<uint16_t> base(n, 0), prefix(n, 1);
+    uint16_t product = 1;
+    for (size_t d = 1; d < n; d++) {
+        prefix[d] = product;
+        product = Gf16::mul(product, static_cast<uint16_t>(1 ^ powers[d]));
+    }
+    uint16_t r = Gf16::inv(product);
+    for (size_t d = n - 1; d >= 1; d--) {
+        uint16_t value = static_cast<uint16_t>(1 ^ powers[d]);
+        base[d] = Gf16::mul(r, prefix[d]);
+        r = Gf16::mul(r, value);
+    }
+    const uint16_t ginv = Gf16::inv(Gf16::kGenerator);
+    std::vector<uint16_t> c(k * m);
+    uint16_t scale = 1;
+    for (size_t i = 0; i < k; i++) {
+        for (size_t j = 0; j < m; j++) c[i * m + j] = Gf16::mul(scale, base[k + j - i]);
+        scale = Gf16::mul(scale, ginv);
+    }
+    return c;
+}
+
+void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t quads,
+                         uint32_t *out) {
+    const Gf256 &f = gf256();
+    for (size_t t = 0; t < T; t++)
+        for (size_t q = 0; q < quads; q++) {
+            uint32_t *tab = out + (t * quads + q) * 32;
+            for (int n = 0; n < 16; n++) {
+                uint32_t lo = 0, hi = 0;
+                for (int s = 0; s < 4; s++) {
+                    size_t r = q * 4 + s;
+                    if (r >= R) continue;
+                    uint8_t c = coef[t * R + r];
+                    lo |= static_cast<uint32_t>(f.m(c, static_cast<uint8_t>(n))) << (8 * s);
+                    hi |= static_cast<uint32_t>(f.m(c, static_cast<uint8_t>(n << 4))) << (8 * s);
+                }
+                tab[n] = lo;
+                tab[16 + n] = hi;
+            }
+        }
+}
+
+}  // namespace scrs
