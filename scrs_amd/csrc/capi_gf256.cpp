@@ -25,7 +25,7 @@ struct scrs_encode_scratch {
 struct scrs_batch_encoder {
     scrs_profile prof{};
     int device = 0;
-    uint32_t quads = 0;
+    uint32_t groups = 0;
     DevBuf d_tables;
     hipStream_t stream = nullptr;
     std::mutex mu;            // guards `own` for scratch-less host-slice calls
@@ -53,7 +53,7 @@ struct scrs_decode_scratch {
 struct scrs_batch_decoder {
     scrs_profile prof{};
     int device = 0;
-    uint32_t rmax = 0, quads = 0;
+    uint32_t rmax = 0, groups = 0;
     DevBuf d_coeffs;
     hipStream_t stream = nullptr;
     DecodeWorkspace ws;          // batched device calls
@@ -90,7 +90,7 @@ scrs_status construction_error(hipError_t err, int32_t engine) {
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 hipError_t ensure_workspace(DecodeWorkspace &ws, uint64_t count, uint32_t k, uint32_t rmax,
-                            uint32_t quads) {
+                            uint32_t groups) {
     hipError_t err = hipSuccess;
     if (!ws.idx_copied) {
         err = hipEventCreateWithFlags(&ws.idx_copied, hipEventDisableTiming);
@@ -99,7 +99,7 @@ hipError_t ensure_workspace(DecodeWorkspace &ws, uint64_t count, uint32_t k, uin
     if (count <= ws.cap) return hipSuccess;
     const size_t kk = k, rr = rmax ? rmax : 1;
     if ((err = ws.d_idx.ensure(count * kk * sizeof(uint16_t))) != hipSuccess) return err;
-    if ((err = ws.d_tables.ensure(count * kk * quads * 32 * sizeof(uint32_t))) != hipSuccess) return err;
+    if ((err = ws.d_tables.ensure(count * kk * groups * 128 * sizeof(uint32_t))) != hipSuccess) return err;
     if ((err = ws.d_rows.ensure(count * sizeof(uint32_t))) != hipSuccess) return err;
     if ((err = ws.d_src.ensure(count * kk * sizeof(void *))) != hipSuccess) return err;
     if ((err = ws.d_dst.ensure(count * rr * sizeof(void *))) != hipSuccess) return err;
@@ -134,7 +134,7 @@ scrs_status enqueue_gf256_decode(const scrs_batch_decoder *dec, DecodeWorkspace 
     const uint32_t k = static_cast<uint32_t>(dec->prof.k), m = static_cast<uint32_t>(dec->prof.m);
     const uint64_t L = dec->prof.symbol_len;
     const uint64_t chunk_max = 65535;
-    hipError_t err = ensure_workspace(ws, count < chunk_max ? count : chunk_max, k, dec->rmax, dec->quads);
+    hipError_t err = ensure_workspace(ws, count < chunk_max ? count : chunk_max, k, dec->rmax, dec->groups);
     if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     for (uint64_t c0 = 0; c0 < count; c0 += chunk_max) {
         const uint64_t cn = count - c0 < chunk_max ? count - c0 : chunk_max;
@@ -161,7 +161,7 @@ scrs_status enqueue_gf256_decode(const scrs_batch_decoder *dec, DecodeWorkspace 
         sa.k = k;
         sa.m = m;
         sa.rmax = dec->rmax;
-        sa.quads = dec->quads;
+        sa.groups = dec->groups;
         sa.L = L;
         sa.count = cn;
         if ((err = launch_decode_solve_gf256(sa, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
@@ -171,10 +171,10 @@ scrs_status enqueue_gf256_decode(const scrs_batch_decoder *dec, DecodeWorkspace 
         ra.copy_ptrs = ws.d_copy.as<uint8_t *const>();
         ra.rows = ws.d_rows.as<uint32_t>();
         ra.tables = ws.d_tables.as<uint32_t>();
-        ra.table_cw_stride = static_cast<uint64_t>(k) * dec->quads * 32;
+        ra.table_cw_stride = static_cast<uint64_t>(k) * dec->groups * 128;
         ra.T = k;
         ra.rmax = dec->rmax;
-        ra.quads = dec->quads;
+        ra.groups = dec->groups;
         ra.L = L;
         ra.count = cn;
         if ((err = launch_rows_terms_lds(ra, vec16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
@@ -247,11 +247,9 @@ extern "C" scrs_status scrs_fill_random_dev(uint8_t *dst_dev, uint64_t len, uint
 scrs_status gf256_encoder_init(scrs_batch_encoder *enc) {
     const scrs_profile &p = enc->prof;
     const std::vector<uint8_t> coeffs = gf256_matrix(p);  // k x m, term-major already
-    enc->quads = static_cast<uint32_t>((p.m + 3) / 4);
-    // Row groups of 16 address quads rg*4 + q: pad to whole groups.
-    enc->quads = (enc->quads + 3) / 4 * 4;
-    std::vector<uint32_t> tables(p.k * enc->quads * 32);
-    build_nibble_tables(coeffs.data(), p.k, p.m, enc->quads, tables.data());
+    enc->groups = static_cast<uint32_t>((p.m + 15) / 16);
+    std::vector<uint32_t> tables(p.k * enc->groups * 128);
+    build_nibble_tables(coeffs.data(), p.k, p.m, enc->groups, tables.data());
     hipError_t err = enc->d_tables.ensure(tables.size() * sizeof(uint32_t));
     if (err != hipSuccess) return construction_error(err, p.engine);
     err = hipMemcpy(enc->d_tables.ptr, tables.data(), tables.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
@@ -403,7 +401,7 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
         a.table_cw_stride = 0;
         a.T = static_cast<uint32_t>(k);
         a.rmax = static_cast<uint32_t>(m);
-        a.quads = enc->quads;
+        a.groups = enc->groups;
         a.L = L;
         a.count = count - c0 < 65535 ? count - c0 : 65535;
         hipError_t err = launch_rows_terms_lds(a, vec16, s);
@@ -429,7 +427,7 @@ extern "C" scrs_status scrs_batch_decoder_create(const scrs_profile *profile, sc
     (void)hipGetDevice(&dec->device);
     const uint64_t rmax = p.k < p.m ? p.k : p.m;
     dec->rmax = static_cast<uint32_t>(rmax);
-    dec->quads = static_cast<uint32_t>((rmax + 15) / 16 * 4);
+    dec->groups = static_cast<uint32_t>((rmax + 15) / 16);
     dec->own.engine = p.engine;
     dec->own.k = p.k;
     dec->own.m = p.m;
@@ -521,7 +519,7 @@ static scrs_status gf256_decode_host(scrs_batch_decoder *dec, const uint64_t *in
         (err = hipMemcpyAsync(sc.ws.d_payload_ptrs.ptr, ptrs.data(), k * sizeof(void *), hipMemcpyHostToDevice, s)) !=
             hipSuccess)
         return st(SCRS_ERR_DEVICE, err);
-    if ((err = ensure_workspace(sc.ws, 1, static_cast<uint32_t>(k), dec->rmax, dec->quads)) != hipSuccess)
+    if ((err = ensure_workspace(sc.ws, 1, static_cast<uint32_t>(k), dec->rmax, dec->groups)) != hipSuccess)
         return st(SCRS_ERR_DEVICE, err);
     if ((err = hipMemsetAsync(sc.ws.d_flag.ptr, 0, sizeof(uint32_t), s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     const bool vec16 = L % 16 == 0;

@@ -4,6 +4,14 @@
 //   dst_r[p] = sum_t C[t][r] * src_t[p]            (GF(2^8), sum = XOR)
 // which is the reference's rows_terms kernel family (scrs/src/simd/rows.rs:257,
 // scrs/src/simd/x86.rs:658) applied to a whole batch of codewords at once.
+//
+// Table path.  For every term t and row group of 16 rows the product table
+// holds 32 entries of 16 bytes: entry n (n < 16) is the row vector
+// (C[t][r] * n) and entry 16 + n is (C[t][r] * (n << 4)), byte r - 16g of the
+// entry for row r.  A source byte x contributes lo[x & 15] ^ hi[x >> 4] to all
+// 16 rows at once, so each byte costs two ds_read_b128 and four XORs
+// regardless of the row count.  Passes with <= 8 or <= 4 rows use 8- or
+// 4-byte entries.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -15,26 +23,14 @@ constexpr int kThreads = 256;                     // 4 wave64 per workgroup
 constexpr int kBytesPerThread = 16;               // one 16-byte load per source
 constexpr int kTile = kThreads * kBytesPerThread; // 4096 columns per workgroup
 constexpr int kTermChunk = 32;                    // terms staged in LDS at once
-constexpr int kRowsPerPass = 16;                  // 4 row quads per pass
+constexpr int kRowsPerPass = 16;                  // one 16-byte table entry
+constexpr int kGroupDwords = 32 * 4;              // one (term, row group) table
 
-// Product tables of one pass: [term][quad][lo 16 | hi 16] dwords = 16 KiB.
-constexpr int kLdsDwords = kTermChunk * 4 * 32;
-
-template <int NQ>
-__device__ __forceinline__ void fold_term(uint32_t (&acc)[4][16], const uint32_t *tt,
-                                          const uint32_t (&w)[4]) {
-#pragma unroll
-    for (int b = 0; b < 16; b++) {
-        const uint32_t byte = (w[b >> 2] >> ((b & 3) * 8)) & 0xffu;
-        const uint32_t lo = byte & 15u, hi = byte >> 4;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) acc[q][b] ^= tt[q * 32 + lo] ^ tt[q * 32 + 16 + hi];
-    }
-}
+// LDS for one chunk at 16-byte entries: 32 terms x 32 entries x 16 B = 16 KiB.
+constexpr int kLdsBytes = kTermChunk * 32 * 16;
 
 template <bool VEC>
-__device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t L,
-                                       uint32_t (&w)[4]) {
+__device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t L, uint32_t (&w)[4]) {
     if (VEC) {
         const uint4 v = *reinterpret_cast<const uint4 *>(src + p);
         w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
@@ -53,8 +49,7 @@ __device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t 
 }
 
 template <bool VEC>
-__device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L,
-                                        const uint32_t (&w)[4]) {
+__device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
     if (VEC) {
         *reinterpret_cast<uint4 *>(dst + p) = make_uint4(w[0], w[1], w[2], w[3]);
     } else {
@@ -68,85 +63,181 @@ __device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L,
     }
 }
 
-template <bool VEC, int NQ>
-__device__ void rows_terms_pass(const RowsTermsArgs &a, uint32_t *lds, uint64_t c, int rg,
-                                int rows_here, uint64_t p, bool do_copy) {
-    uint32_t acc[4][16];
+// Entry of RW bytes at byte offset `off` of the LDS chunk.
+template <int RW> struct Entry;
+template <> struct Entry<16> {
+    uint32_t d[4];
+    __device__ __forceinline__ void load(const uint8_t *lds, uint32_t off) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(lds + off);
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+};
+template <> struct Entry<8> {
+    uint32_t d[2];
+    __device__ __forceinline__ void load(const uint8_t *lds, uint32_t off) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(lds + off);
+        d[0] = v.x; d[1] = v.y;
+    }
+};
+template <> struct Entry<4> {
+    uint32_t d[1];
+    __device__ __forceinline__ void load(const uint8_t *lds, uint32_t off) {
+        d[0] = *reinterpret_cast<const uint32_t *>(lds + off);
+    }
+};
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// Fold one 16-byte source slice into the accumulators.  `tab` points at the
+// term's 32 entries (lo 0..15, hi 16..31) of RW bytes each.
+template <int RW>
+__device__ __forceinline__ void fold_term(uint32_t (&acc)[16][RW / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
+    constexpr int SH = RW == 16 ? 4 : (RW == 8 ? 3 : 2);  // log2(RW)
+    constexpr uint32_t MASK = 0x0F0F0F0Fu << SH;
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int j = 0; j < 4; j++) {
+        // Byte b of `lo` / `hi` is the entry byte offset of that column's nibble.
+        const uint32_t lo = (w[j] << SH) & MASK;
+        const uint32_t hi = (SH == 4) ? (w[j] & MASK) : ((w[j] >> (4 - SH)) & MASK);
 #pragma unroll
-        for (int b = 0; b < 16; b++) acc[q][b] = 0;
+        for (int b = 0; b < 4; b++) {
+            Entry<RW> el, eh;
+            el.load(tab, __builtin_amdgcn_ubfe(lo, 8 * b, 8));
+            eh.load(tab + 16 * RW, __builtin_amdgcn_ubfe(hi, 8 * b, 8));
+#pragma unroll
+            for (int d = 0; d < RW / 4; d++) acc[4 * j + b][d] = xor3(acc[4 * j + b][d], el.d[d], eh.d[d]);
+        }
+    }
+}
+
+// One pass over all terms for up to 16 rows of one codeword.  DEC selects
+// the decode addressing: per-codeword source / destination pointer tables
+// and verbatim copies of the present data rows; otherwise sources and
+// destinations are strided rows of back-to-back codewords (encode).
+template <bool VEC, bool DEC, int RW>
+__device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here,
+                                uint64_t p, bool do_copy) {
+    constexpr int D = RW / 4;
+    uint32_t acc[16][D];
+#pragma unroll
+    for (int b = 0; b < 16; b++)
+#pragma unroll
+        for (int d = 0; d < D; d++) acc[b][d] = 0;
 
     const bool active = p < a.L;
     const uint32_t *tab_cw = a.tables + c * a.table_cw_stride;
+    const uint8_t *src_cw = DEC ? nullptr : a.src_base + c * a.src_cw_stride;
     for (uint32_t t0 = 0; t0 < a.T; t0 += kTermChunk) {
         const uint32_t tc = min(static_cast<uint32_t>(kTermChunk), a.T - t0);
-        if (NQ > 0) {
-            __syncthreads();
-            // Stage the product tables of this term chunk and row group.
-            constexpr uint32_t per_term = (NQ > 0 ? NQ : 1) * 32;
-            for (uint32_t i = threadIdx.x; i < tc * per_term; i += kThreads) {
-                const uint32_t t = i / per_term, rem = i % per_term;
-                const uint32_t q = rem >> 5, n = rem & 31;
-                lds[(t * 4 + q) * 32 + n] = tab_cw[((t0 + t) * a.quads + rg * 4 + q) * 32 + n];
-            }
-            __syncthreads();
+        __syncthreads();
+        // Stage this chunk's tables: entry n of term t keeps its first RW bytes.
+        for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += kThreads) {
+            const uint32_t t = i / (32 * D), rem = i % (32 * D);
+            const uint32_t n = rem / D, d = rem % D;
+            reinterpret_cast<uint32_t *>(lds)[i] =
+                tab_cw[((t0 + t) * a.groups + rg) * kGroupDwords + n * 4 + d];
         }
+        __syncthreads();
         if (!active) continue;
-#pragma unroll 2
-        for (uint32_t t = 0; t < tc; t++) {
-            const uint64_t gt = t0 + t;
-            const uint8_t *src = a.src_ptrs ? a.src_ptrs[c * a.T + gt]
-                                            : a.src_base + c * a.src_cw_stride + gt * a.L;
-            uint32_t w[4];
-            load16<VEC>(src, p, a.L, w);
-            if (do_copy) {
-                uint8_t *cp = a.copy_ptrs[c * a.T + gt];
-                if (cp) store16<VEC>(cp, p, a.L, w);
+        if (!DEC && VEC && tc == kTermChunk) {
+            // Fully unrolled so every table address is a compile-time LDS
+            // offset.  Sources stream in groups of kAhead terms: the next
+            // group's loads are in flight while the current group folds.
+            constexpr int kAhead = 4;
+            constexpr int kGroups = kTermChunk / kAhead;
+            // Uniform row base + 32-bit lane offset: the loads take the
+            // scalar-base addressing form.
+            const uint32_t p32 = static_cast<uint32_t>(p);
+            auto ld = [&](int t, uint32_t(&w)[4]) {
+                const uint8_t *row = src_cw + static_cast<uint64_t>(t0 + t) * a.L;
+                const uint4 v = *reinterpret_cast<const uint4 *>(row + p32);
+                w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+            };
+            uint32_t wa[kAhead][4], wb[kAhead][4];
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) ld(u, wa[u]);
+#pragma unroll
+            for (int g = 0; g < kGroups; g += 2) {
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) ld((g + 1) * kAhead + u, wb[u]);
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) fold_term<RW>(acc, lds + (g * kAhead + u) * 32 * RW, wa[u]);
+                __builtin_amdgcn_sched_barrier(0);
+                if (g + 2 < kGroups) {
+#pragma unroll
+                    for (int u = 0; u < kAhead; u++) ld((g + 2) * kAhead + u, wa[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kAhead; u++)
+                    fold_term<RW>(acc, lds + ((g + 1) * kAhead + u) * 32 * RW, wb[u]);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            fold_term<NQ>(acc, lds + t * 128, w);
+        } else {
+#pragma unroll 2
+            for (uint32_t t = 0; t < tc; t++) {
+                const uint64_t gt = t0 + t;
+                const uint8_t *src = DEC ? a.src_ptrs[c * a.T + gt] : src_cw + gt * a.L;
+                uint32_t w[4];
+                load16<VEC>(src, p, a.L, w);
+                if (DEC && do_copy) {
+                    uint8_t *cp = a.copy_ptrs[c * a.T + gt];
+                    if (cp) store16<VEC>(cp, p, a.L, w);
+                }
+                fold_term<RW>(acc, lds + t * 32 * RW, w);
+            }
         }
     }
     if (!active) return;
 #pragma unroll
-    for (int q = 0; q < NQ; q++)
+    for (int r = 0; r < RW; r++) {
+        if (r >= rows_here) continue;
+        const int row = rg * kRowsPerPass + r;
+        uint8_t *dst = DEC ? a.dst_ptrs[c * a.rmax + row]
+                           : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(row) * a.L;
+        const int d = r >> 2, s = (r & 3) * 8;
+        uint32_t o[4];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int r = rg * kRowsPerPass + q * 4 + s;
-            if (q * 4 + s >= rows_here) continue;
-            uint8_t *dst = a.dst_ptrs ? a.dst_ptrs[c * a.rmax + r]
-                                      : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(r) * a.L;
-            uint32_t o[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                o[j] = ((acc[q][4 * j + 0] >> (8 * s)) & 0xffu) |
-                       (((acc[q][4 * j + 1] >> (8 * s)) & 0xffu) << 8) |
-                       (((acc[q][4 * j + 2] >> (8 * s)) & 0xffu) << 16) |
-                       (((acc[q][4 * j + 3] >> (8 * s)) & 0xffu) << 24);
-            }
-            store16<VEC>(dst, p, a.L, o);
+        for (int j = 0; j < 4; j++) {
+            o[j] = ((acc[4 * j + 0][d] >> s) & 0xffu) | (((acc[4 * j + 1][d] >> s) & 0xffu) << 8) |
+                   (((acc[4 * j + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * j + 3][d] >> s) & 0xffu) << 24);
         }
+        store16<VEC>(dst, p, a.L, o);
+    }
 }
 
 template <bool VEC>
+__device__ void copy_only_pass(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
+    if (p >= a.L) return;
+    for (uint32_t t = 0; t < a.T; t++) {
+        uint8_t *cp = a.copy_ptrs[c * a.T + t];
+        if (!cp) continue;
+        uint32_t w[4];
+        load16<VEC>(a.src_ptrs[c * a.T + t], p, a.L, w);
+        store16<VEC>(cp, p, a.L, w);
+    }
+}
+
+template <bool VEC, bool DEC>
 __global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs a) {
-    __shared__ uint32_t lds[kLdsDwords];
+    __shared__ __align__(16) uint8_t lds[kLdsBytes];
     const uint64_t c = blockIdx.y;
     const int rg = blockIdx.z;
-    const uint32_t R = a.rows ? a.rows[c] : a.rmax;
-    const int rows_here = static_cast<int>(min(static_cast<uint32_t>(kRowsPerPass),
-                                               R > static_cast<uint32_t>(rg * kRowsPerPass)
-                                                   ? R - rg * kRowsPerPass : 0u));
-    const bool do_copy = a.copy_ptrs != nullptr && rg == 0;
+    const uint32_t R = DEC ? a.rows[c] : a.rmax;
+    const uint32_t base = static_cast<uint32_t>(rg) * kRowsPerPass;
+    const int rows_here = static_cast<int>(R > base ? min(static_cast<uint32_t>(kRowsPerPass), R - base) : 0u);
+    const bool do_copy = DEC && rg == 0;
     if (rows_here == 0 && !do_copy) return;
     const uint64_t p = static_cast<uint64_t>(blockIdx.x) * kTile + threadIdx.x * kBytesPerThread;
-    const int nq = (rows_here + 3) / 4;
-    switch (nq) {
-        case 0: rows_terms_pass<VEC, 0>(a, lds, c, rg, rows_here, p, do_copy); break;
-        case 1: rows_terms_pass<VEC, 1>(a, lds, c, rg, rows_here, p, do_copy); break;
-        case 2: rows_terms_pass<VEC, 2>(a, lds, c, rg, rows_here, p, do_copy); break;
-        case 3: rows_terms_pass<VEC, 3>(a, lds, c, rg, rows_here, p, do_copy); break;
-        default: rows_terms_pass<VEC, 4>(a, lds, c, rg, rows_here, p, do_copy); break;
+    if (DEC && rows_here == 0) {
+        copy_only_pass<VEC>(a, c, p);
+    } else if (rows_here <= 4) {
+        rows_terms_pass<VEC, DEC, 4>(a, lds, c, rg, rows_here, p, do_copy);
+    } else if (rows_here <= 8) {
+        rows_terms_pass<VEC, DEC, 8>(a, lds, c, rg, rows_here, p, do_copy);
+    } else {
+        rows_terms_pass<VEC, DEC, 16>(a, lds, c, rg, rows_here, p, do_copy);
     }
 }
 
@@ -165,7 +256,7 @@ __device__ __forceinline__ uint8_t ginv(const uint8_t *exp, const uint8_t *log, 
 // GF(2^8) exp (512 entries, doubled) and log tables.
 __constant__ uint8_t c_exp[512];
 __constant__ uint8_t c_log[256];
-bool g_tables_ready = false;
+int g_tables_device = -1;
 
 __global__ __launch_bounds__(64) void decode_solve_gf256_kernel(DecodeSolveArgs a) {
     extern __shared__ uint8_t smem[];
@@ -175,7 +266,8 @@ __global__ __launch_bounds__(64) void decode_solve_gf256_kernel(DecodeSolveArgs 
     uint8_t *rc = seen + 256;       // 256 repair columns, received order
     uint8_t *ms = rc + 256;         // 256 missing data rows, ascending
     uint8_t *tp = ms + 256;         // 256 position of received symbol t among the repairs
-    uint8_t *aug = tp + 256;        // rmax x (2 * rmax) augmented matrix
+    uint8_t *fused = tp + 256;      // rmax x k fused coefficients
+    uint8_t *aug = fused + a.rmax * a.k;  // rmax x (2 * rmax) augmented matrix
     __shared__ uint32_t s_e;
     __shared__ uint32_t s_piv;
 
@@ -202,7 +294,8 @@ __global__ __launch_bounds__(64) void decode_solve_gf256_kernel(DecodeSolveArgs 
     __syncthreads();
     const uint32_t e = s_e;
     const uint32_t w2 = 2 * e;
-    // aug[t][s] = C[ms[s]][rc[t]], aug[t][e + s] = identity.
+    // aug[t][s] = C[ms[s]][rc[t]] (the reference's B, batch/codec.rs:383-389),
+    // aug[t][e + s] = identity.
     for (uint32_t x = lane; x < e * w2; x += 64) {
         const uint32_t t = x / w2, s = x % w2;
         aug[t * w2 + s] = s < e ? a.coeffs[ms[s] * m + rc[t]] : static_cast<uint8_t>(s - e == t);
@@ -243,30 +336,35 @@ __global__ __launch_bounds__(64) void decode_solve_gf256_kernel(DecodeSolveArgs 
             if (r != col) aug[r * w2 + col] = 0;
         __syncthreads();
     }
-    // Binv[s][t] = aug[s][e + t].  The fused coefficient of received symbol
-    // t' for missing row s is Binv[s][pos] for a repair and
-    // sum_u Binv[s][u] * C[i][rc[u]] for present data row i; each is
-    // expanded straight into the packed product tables.
-    uint32_t *tab = a.tables + c * static_cast<uint64_t>(k) * a.quads * 32;
-    for (uint32_t x = lane; x < k * a.quads * 32; x += 64) {
-        const uint32_t t = x / (a.quads * 32), rem = x % (a.quads * 32);
-        const uint32_t q = rem >> 5, n = rem & 31;
+    // Binv[s][u] = aug[s][e + u].  Fused coefficient of received symbol t for
+    // missing row s: Binv[s][pos] for a repair, sum_u Binv[s][u] * C[i][rc[u]]
+    // for a present data row i (the subtract and apply passes of
+    // batch/codec.rs:376-404 folded into one linear map).
+    for (uint32_t x = lane; x < e * k; x += 64) {
+        const uint32_t s = x / k, t = x % k;
         const uint32_t i = idx[t];
+        uint8_t f;
+        if (i >= k) {
+            f = aug[s * w2 + e + tp[t]];
+        } else {
+            f = 0;
+            for (uint32_t u = 0; u < e; u++) f ^= gmul(exp, log, aug[s * w2 + e + u], a.coeffs[i * m + rc[u]]);
+        }
+        fused[s * k + t] = f;
+    }
+    __syncthreads();
+    // Product tables: [t][g][n][q] dwords, byte s4 of dword q = row 16g+4q+s4.
+    uint32_t *tab = a.tables + c * static_cast<uint64_t>(k) * a.groups * kGroupDwords;
+    for (uint32_t x = lane; x < k * a.groups * kGroupDwords; x += 64) {
+        const uint32_t t = x / (a.groups * kGroupDwords), rem = x % (a.groups * kGroupDwords);
+        const uint32_t g = rem / kGroupDwords, nq = rem % kGroupDwords;
+        const uint32_t n = nq >> 2, q = nq & 3;
         const uint8_t nib = static_cast<uint8_t>(n < 16 ? n : (n - 16) << 4);
         uint32_t word = 0;
         for (uint32_t s4 = 0; s4 < 4; s4++) {
-            const uint32_t s = q * 4 + s4;
+            const uint32_t s = g * 16 + q * 4 + s4;
             if (s >= e) break;
-            uint8_t f;
-            if (i >= k) {
-                // Received repair: its position among the received repairs.
-                f = aug[s * w2 + e + tp[t]];
-            } else {
-                f = 0;
-                for (uint32_t u = 0; u < e; u++)
-                    f ^= gmul(exp, log, aug[s * w2 + e + u], a.coeffs[i * m + rc[u]]);
-            }
-            word |= static_cast<uint32_t>(gmul(exp, log, f, nib)) << (8 * s4);
+            word |= static_cast<uint32_t>(gmul(exp, log, fused[s * k + t], nib)) << (8 * s4);
         }
         tab[x] = word;
     }
@@ -307,17 +405,28 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &a, bool vec16, hipStream_t
     if (a.count == 0 || a.L == 0) return hipSuccess;
     const uint64_t tiles = (a.L + kTile - 1) / kTile;
     const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
-    if (tiles > 0x7fffffffull || a.count > 65535 || groups > 65535) return hipErrorInvalidValue;
+    if (tiles > 0x7fffffffull || a.count > 65535 || groups > 65535 || groups > a.groups + (a.rmax == 0))
+        return hipErrorInvalidValue;
     dim3 grid(static_cast<uint32_t>(tiles), static_cast<uint32_t>(a.count), groups);
-    if (vec16)
-        hipLaunchKernelGGL(rows_terms_lds_kernel<true>, grid, dim3(kThreads), 0, s, a);
+    const bool dec = a.src_ptrs != nullptr;
+    if (dec != (a.dst_ptrs != nullptr) || dec != (a.copy_ptrs != nullptr) || dec != (a.rows != nullptr))
+        return hipErrorInvalidValue;
+    if (vec16 && dec)
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true>), grid, dim3(kThreads), 0, s, a);
+    else if (vec16)
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false>), grid, dim3(kThreads), 0, s, a);
+    else if (dec)
+        hipLaunchKernelGGL((rows_terms_lds_kernel<false, true>), grid, dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL(rows_terms_lds_kernel<false>, grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<false, false>), grid, dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
 static hipError_t ensure_field_tables() {
-    if (g_tables_ready) return hipSuccess;
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (g_tables_device == dev) return hipSuccess;
     uint8_t exp[512], log[256] = {0};
     uint8_t v = 1;
     for (int i = 0; i < 255; i++) {
@@ -327,10 +436,10 @@ static hipError_t ensure_field_tables() {
     }
     exp[510] = exp[0];
     exp[511] = exp[1];
-    hipError_t err = hipMemcpyToSymbol(HIP_SYMBOL(c_exp), exp, sizeof(exp));
+    err = hipMemcpyToSymbol(HIP_SYMBOL(c_exp), exp, sizeof(exp));
     if (err != hipSuccess) return err;
     err = hipMemcpyToSymbol(HIP_SYMBOL(c_log), log, sizeof(log));
-    if (err == hipSuccess) g_tables_ready = true;
+    if (err == hipSuccess) g_tables_device = dev;
     return err;
 }
 
@@ -338,13 +447,13 @@ hipError_t launch_decode_solve_gf256(const DecodeSolveArgs &a, hipStream_t s) {
     if (a.count == 0) return hipSuccess;
     hipError_t err = ensure_field_tables();
     if (err != hipSuccess) return err;
-    const size_t shmem = 512 + 256 * 5 + static_cast<size_t>(a.rmax) * 2 * a.rmax;
+    const size_t shmem = 512 + 256 * 5 + static_cast<size_t>(a.rmax) * a.k + static_cast<size_t>(a.rmax) * 2 * a.rmax;
+    if (shmem > 64 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(decode_solve_gf256_kernel, dim3(static_cast<uint32_t>(a.count)), dim3(64), shmem, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_splitmix(uint8_t *dst, uint64_t len, uint64_t seed, uint64_t byte_offset,
-                                hipStream_t s) {
+hipError_t launch_fill_splitmix(uint8_t *dst, uint64_t len, uint64_t seed, uint64_t byte_offset, hipStream_t s) {
     if (len == 0) return hipSuccess;
     if (byte_offset % 8) return hipErrorInvalidValue;
     const uint64_t words = (len + 7) / 8;

@@ -25,11 +25,11 @@ struct RowsTermsArgs {
     uint8_t *const *dst_ptrs = nullptr;
     uint8_t *const *copy_ptrs = nullptr;
     const uint32_t *rows = nullptr;   // per-codeword row count, or null for `rmax`
-    const uint32_t *tables = nullptr; // packed nibble tables (build_nibble_tables layout)
+    const uint32_t *tables = nullptr; // product tables (build_nibble_tables layout)
     uint64_t table_cw_stride = 0;     // dwords between codewords' tables, 0 = shared
     uint32_t T = 0;                   // terms per codeword
     uint32_t rmax = 0;                // rows (upper bound when `rows` is set)
-    uint32_t quads = 0;               // row quads per term in the tables
+    uint32_t groups = 0;              // 16-row groups per term in the tables
     uint64_t L = 0;                   // symbol length in bytes
     uint64_t count = 0;               // codewords
 };
@@ -50,13 +50,13 @@ struct DecodeSolveArgs {
     const uint8_t *data_base = nullptr;  // layout mode: data rows c*k*L + i*L
     const uint8_t *repair_base = nullptr;// layout mode: repair rows c*m*L + j*L
     uint8_t *out_base = nullptr;         // out rows c*k*L + i*L
-    uint32_t *tables = nullptr;          // [count][k][quads][32]
+    uint32_t *tables = nullptr;          // [count][k][groups][32][4]
     uint32_t *rows = nullptr;            // [count]
     const uint8_t **src_ptrs = nullptr;  // [count][k]
     uint8_t **dst_ptrs = nullptr;        // [count][rmax]
     uint8_t **copy_ptrs = nullptr;       // [count][k]
     uint32_t *error_flag = nullptr;      // set non-zero if a submatrix is singular
-    uint32_t k = 0, m = 0, rmax = 0, quads = 0;
+    uint32_t k = 0, m = 0, rmax = 0, groups = 0;
     uint64_t L = 0, count = 0;
 };
 

@@ -61,24 +61,19 @@ std::vector<uint16_t> tower_cauchy_matrix(size_t k, size_t m) {
     return c;
 }
 
-void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t quads,
+void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t groups,
                          uint32_t *out) {
     const Gf256 &f = gf256();
     for (size_t t = 0; t < T; t++)
-        for (size_t q = 0; q < quads; q++) {
-            uint32_t *tab = out + (t * quads + q) * 32;
-            for (int n = 0; n < 16; n++) {
-                uint32_t lo = 0, hi = 0;
-                for (int s = 0; s < 4; s++) {
-                    size_t r = q * 4 + s;
-                    if (r >= R) continue;
-                    uint8_t c = coef[t * R + r];
-                    lo |= static_cast<uint32_t>(f.m(c, static_cast<uint8_t>(n))) << (8 * s);
-                    hi |= static_cast<uint32_t>(f.m(c, static_cast<uint8_t>(n << 4))) << (8 * s);
+        for (size_t g = 0; g < groups; g++) {
+            uint8_t *tab = reinterpret_cast<uint8_t *>(out + (t * groups + g) * 128);
+            for (int n = 0; n < 16; n++)
+                for (int s = 0; s < 16; s++) {
+                    const size_t r = g * 16 + s;
+                    const uint8_t c = r < R ? coef[t * R + r] : 0;
+                    tab[n * 16 + s] = f.m(c, static_cast<uint8_t>(n));
+                    tab[(16 + n) * 16 + s] = f.m(c, static_cast<uint8_t>(n << 4));
                 }
-                tab[n] = lo;
-                tab[16 + n] = hi;
-            }
         }
 }
 

@@ -26,12 +26,13 @@ std::vector<uint8_t> good_cauchy_matrix(size_t k, size_t m);
 // batch inversion (tower/cauchy.rs:69-137).
 std::vector<uint16_t> tower_cauchy_matrix(size_t k, size_t m);
 
-// Packed split-nibble product tables for a T x R coefficient block given
-// term-major (coef[t*R + r]).  For term t and row quad q the 32 dwords at
-// ((t * quads) + q) * 32 hold lo[n] (n < 16) and hi[n] (16 + n); byte s of
-// each dword is the product for row 4q + s: coef * n and coef * (n << 4).
-// Rows at or beyond R are zero.  `quads` >= ceil(R / 4).
-void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t quads,
+// Split-nibble product tables for a T x R coefficient block given term-major
+// (coef[t*R + r]).  For term t and 16-row group g, the 512 bytes at dword
+// (t * groups + g) * 128 hold 32 entries of 16 bytes: entry n (n < 16) is
+// the row vector coef[t][16g + s] * n, entry 16 + n is coef * (n << 4); byte
+// s of an entry belongs to row 16g + s.  Rows at or beyond R are zero.
+// `groups` >= ceil(R / 16).
+void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t groups,
                          uint32_t *out);
 
 }  // namespace scrs
