@@ -95,28 +95,46 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 template <int RW>
 __device__ __forceinline__ void fold_term(uint32_t (&acc)[16][RW / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
     constexpr int SH = RW == 16 ? 4 : (RW == 8 ? 3 : 2);  // log2(RW)
-    constexpr uint32_t MASK = 0x0F0F0F0Fu << SH;
+    constexpr uint32_t BMASK = 0x0Fu << SH;                 // one entry offset
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        // Byte b of `lo` / `hi` is the entry byte offset of that column's nibble.
-        const uint32_t lo = (w[j] << SH) & MASK;
-        const uint32_t hi = (SH == 4) ? (w[j] & MASK) : ((w[j] >> (4 - SH)) & MASK);
+        // Rotations place each byte's low (high) nibble times RW into bits
+        // SH..SH+3 of that byte; masking byte b gives the entry offset.
+        const uint32_t lo = __builtin_amdgcn_alignbit(w[j], w[j], 32 - SH);
+        const uint32_t hi = SH == 4 ? w[j] : __builtin_amdgcn_alignbit(w[j], w[j], 4 - SH);
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             Entry<RW> el, eh;
-            el.load(tab, __builtin_amdgcn_ubfe(lo, 8 * b, 8));
-            eh.load(tab + 16 * RW, __builtin_amdgcn_ubfe(hi, 8 * b, 8));
+            el.load(tab, (lo >> (8 * b)) & BMASK);
+            eh.load(tab + 16 * RW, (hi >> (8 * b)) & BMASK);
 #pragma unroll
             for (int d = 0; d < RW / 4; d++) acc[4 * j + b][d] = xor3(acc[4 * j + b][d], el.d[d], eh.d[d]);
         }
+        // Fold four columns at a time: keeps the lookups in flight bounded
+        // and the register budget at four waves per SIMD.
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
+
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
+// Entries of the per-codeword pointer / row-count tables are the same for
+// every lane and the kernel never writes those tables: read them through the
+// constant address space so they become scalar loads into SGPRs.
+typedef __attribute__((address_space(4))) const uint64_t *const_u64_ptr;
+typedef __attribute__((address_space(4))) const uint32_t *const_u32_ptr;
+
+template <typename T>
+__device__ __forceinline__ T *load_uniform(T *const *ptr) {
+    return reinterpret_cast<T *>(*(const_u64_ptr)(ptr));
+}
+__device__ __forceinline__ uint32_t load_uniform(const uint32_t *ptr) { return *(const_u32_ptr)(ptr); }
 
 // One pass over all terms for up to 16 rows of one codeword.  DEC selects
 // the decode addressing: per-codeword source / destination pointer tables
 // and verbatim copies of the present data rows; otherwise sources and
 // destinations are strided rows of back-to-back codewords (encode).
-template <bool VEC, bool DEC, int RW>
+template <bool VEC, bool DEC, bool FULL, int RW>
 __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here,
                                 uint64_t p, bool do_copy) {
     constexpr int D = RW / 4;
@@ -141,19 +159,35 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
         }
         __syncthreads();
         if (!active) continue;
-        if (!DEC && VEC && tc == kTermChunk) {
+        if (FULL || (VEC && tc == kTermChunk)) {
             // Fully unrolled so every table address is a compile-time LDS
             // offset.  Sources stream in groups of kAhead terms: the next
             // group's loads are in flight while the current group folds.
             constexpr int kAhead = 4;
             constexpr int kGroups = kTermChunk / kAhead;
-            // Uniform row base + 32-bit lane offset: the loads take the
-            // scalar-base addressing form.
             const uint32_t p32 = static_cast<uint32_t>(p);
+            // Uniform row base + 32-bit lane offset.
+            auto row_of = [&](int t) -> const uint8_t * {
+                if (DEC) return load_uniform(a.src_ptrs + c * a.T + t0 + t);
+                return src_cw + static_cast<uint64_t>(t0 + t) * a.L;
+            };
             auto ld = [&](int t, uint32_t(&w)[4]) {
-                const uint8_t *row = src_cw + static_cast<uint64_t>(t0 + t) * a.L;
-                const uint4 v = *reinterpret_cast<const uint4 *>(row + p32);
+                const uint4 v = *reinterpret_cast<const uint4 *>(row_of(t) + p32);
                 w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+            };
+            // Decode writes each present data row verbatim as it streams by.
+            // The store's buffer range is L for those terms and zero for
+            // received repairs, whose stores the hardware then drops: no
+            // per-term branch in the unrolled stream.
+            const uint32_t copy_range = do_copy ? static_cast<uint32_t>(a.L) : 0u;
+            auto cp = [&](int t, const uint32_t(&w)[4]) {
+                if (!DEC) return;
+                uint8_t *dst = load_uniform(a.copy_ptrs + c * a.T + t0 + t);
+                const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+                    dst ? dst : const_cast<uint8_t *>(row_of(t)), static_cast<short>(0), dst ? copy_range : 0u,
+                    0x00020000);
+                const v4u32 v = {w[0], w[1], w[2], w[3]};
+                __builtin_amdgcn_raw_buffer_store_b128(v, r, p32, 0, 0);
             };
             uint32_t wa[kAhead][4], wb[kAhead][4];
 #pragma unroll
@@ -163,26 +197,31 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
 #pragma unroll
                 for (int u = 0; u < kAhead; u++) ld((g + 1) * kAhead + u, wb[u]);
 #pragma unroll
-                for (int u = 0; u < kAhead; u++) fold_term<RW>(acc, lds + (g * kAhead + u) * 32 * RW, wa[u]);
+                for (int u = 0; u < kAhead; u++) {
+                    cp(g * kAhead + u, wa[u]);
+                    fold_term<RW>(acc, lds + (g * kAhead + u) * 32 * RW, wa[u]);
+                }
                 __builtin_amdgcn_sched_barrier(0);
                 if (g + 2 < kGroups) {
 #pragma unroll
                     for (int u = 0; u < kAhead; u++) ld((g + 2) * kAhead + u, wa[u]);
                 }
 #pragma unroll
-                for (int u = 0; u < kAhead; u++)
+                for (int u = 0; u < kAhead; u++) {
+                    cp((g + 1) * kAhead + u, wb[u]);
                     fold_term<RW>(acc, lds + ((g + 1) * kAhead + u) * 32 * RW, wb[u]);
+                }
                 __builtin_amdgcn_sched_barrier(0);
             }
         } else {
-#pragma unroll 2
+#pragma unroll 1
             for (uint32_t t = 0; t < tc; t++) {
                 const uint64_t gt = t0 + t;
-                const uint8_t *src = DEC ? a.src_ptrs[c * a.T + gt] : src_cw + gt * a.L;
+                const uint8_t *src = DEC ? load_uniform(a.src_ptrs + c * a.T + gt) : src_cw + gt * a.L;
                 uint32_t w[4];
                 load16<VEC>(src, p, a.L, w);
                 if (DEC && do_copy) {
-                    uint8_t *cp = a.copy_ptrs[c * a.T + gt];
+                    uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + gt);
                     if (cp) store16<VEC>(cp, p, a.L, w);
                 }
                 fold_term<RW>(acc, lds + t * 32 * RW, w);
@@ -194,7 +233,7 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
     for (int r = 0; r < RW; r++) {
         if (r >= rows_here) continue;
         const int row = rg * kRowsPerPass + r;
-        uint8_t *dst = DEC ? a.dst_ptrs[c * a.rmax + row]
+        uint8_t *dst = DEC ? load_uniform(a.dst_ptrs + c * a.rmax + row)
                            : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(row) * a.L;
         const int d = r >> 2, s = (r & 3) * 8;
         uint32_t o[4];
@@ -211,20 +250,22 @@ template <bool VEC>
 __device__ void copy_only_pass(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
     if (p >= a.L) return;
     for (uint32_t t = 0; t < a.T; t++) {
-        uint8_t *cp = a.copy_ptrs[c * a.T + t];
+        uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + t);
         if (!cp) continue;
         uint32_t w[4];
-        load16<VEC>(a.src_ptrs[c * a.T + t], p, a.L, w);
+        load16<VEC>(load_uniform(a.src_ptrs + c * a.T + t), p, a.L, w);
         store16<VEC>(cp, p, a.L, w);
     }
 }
 
-template <bool VEC, bool DEC>
+// FULL: the term count is a multiple of the LDS chunk and rows are 16-byte
+// aligned, so only the fully unrolled stream is compiled.
+template <bool VEC, bool DEC, bool FULL>
 __global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs a) {
     __shared__ __align__(16) uint8_t lds[kLdsBytes];
     const uint64_t c = blockIdx.y;
     const int rg = blockIdx.z;
-    const uint32_t R = DEC ? a.rows[c] : a.rmax;
+    const uint32_t R = DEC ? load_uniform(a.rows + c) : a.rmax;
     const uint32_t base = static_cast<uint32_t>(rg) * kRowsPerPass;
     const int rows_here = static_cast<int>(R > base ? min(static_cast<uint32_t>(kRowsPerPass), R - base) : 0u);
     const bool do_copy = DEC && rg == 0;
@@ -233,11 +274,11 @@ __global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs 
     if (DEC && rows_here == 0) {
         copy_only_pass<VEC>(a, c, p);
     } else if (rows_here <= 4) {
-        rows_terms_pass<VEC, DEC, 4>(a, lds, c, rg, rows_here, p, do_copy);
+        rows_terms_pass<VEC, DEC, FULL, 4>(a, lds, c, rg, rows_here, p, do_copy);
     } else if (rows_here <= 8) {
-        rows_terms_pass<VEC, DEC, 8>(a, lds, c, rg, rows_here, p, do_copy);
+        rows_terms_pass<VEC, DEC, FULL, 8>(a, lds, c, rg, rows_here, p, do_copy);
     } else {
-        rows_terms_pass<VEC, DEC, 16>(a, lds, c, rg, rows_here, p, do_copy);
+        rows_terms_pass<VEC, DEC, FULL, 16>(a, lds, c, rg, rows_here, p, do_copy);
     }
 }
 
@@ -411,14 +452,19 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &a, bool vec16, hipStream_t
     const bool dec = a.src_ptrs != nullptr;
     if (dec != (a.dst_ptrs != nullptr) || dec != (a.copy_ptrs != nullptr) || dec != (a.rows != nullptr))
         return hipErrorInvalidValue;
-    if (vec16 && dec)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true>), grid, dim3(kThreads), 0, s, a);
+    const bool full = vec16 && a.T % kTermChunk == 0;
+    if (full && dec)
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, true>), grid, dim3(kThreads), 0, s, a);
+    else if (full)
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false, true>), grid, dim3(kThreads), 0, s, a);
+    else if (vec16 && dec)
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, false>), grid, dim3(kThreads), 0, s, a);
     else if (vec16)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false, false>), grid, dim3(kThreads), 0, s, a);
     else if (dec)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<false, true>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<false, true, false>), grid, dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL((rows_terms_lds_kernel<false, false>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<false, false, false>), grid, dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
