@@ -6,7 +6,9 @@ codeword from the k lowest surviving indices after erasing a seeded uniform
 subset of 1..16 of its 48 indices.  Throughput counts algorithmic bytes:
 (k+m)*L per encoded codeword plus 2*k*L per decoded codeword.
 
-Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
+Prints one JSON line: whole-job throughput, per-phase kernel times measured
+with HIP events on the launch stream, the encode launch against the HBM
+roofline, and the CPU port of the reference algorithm timed on this host.
 """
 from __future__ import annotations
 
@@ -82,9 +84,12 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--count", type=int, default=COUNT, help="codewords per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="before the timed steps, run a plain 1 GiB device copy (torch copy_) so a "
+                         "counter profile can scale FETCH_SIZE / WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the workload shards by "
@@ -106,6 +111,14 @@ def main() -> None:
     fill_random_dev(data, data.numel(), SEED_DATA, 0, stream)
     idx = erasure_indices(count, K, M, 16, SEED_ERASE)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        gib = 1 << 30
+        src_c = torch.ones(gib, dtype=torch.uint8, device="cuda")
+        dst_c = torch.empty_like(src_c)
+        for _ in range(2):
+            dst_c.copy_(src_c)
+        torch.cuda.synchronize()
+        del src_c, dst_c
 
     def step():
         enc.encode_batch_dev(data, rep, count, stream)

@@ -25,6 +25,7 @@ constexpr int kTile = kThreads * kBytesPerThread; // 4096 columns per workgroup
 constexpr int kTermChunk = 32;                    // terms staged in LDS at once
 constexpr int kRowsPerPass = 16;                  // one 16-byte table entry
 constexpr int kGroupDwords = 32 * 4;              // one (term, row group) table
+constexpr uint32_t kXcds = 8;                     // MI355X: 8 XCDs, each with its own L2
 
 // LDS for one chunk at 16-byte entries: 32 terms x 32 entries x 16 B = 16 KiB.
 constexpr int kLdsBytes = kTermChunk * 32 * 16;
@@ -261,16 +262,24 @@ __device__ void copy_only_pass(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
 // FULL: the term count is a multiple of the LDS chunk and rows are 16-byte
 // aligned, so only the fully unrolled stream is compiled.
 template <bool VEC, bool DEC, bool FULL>
-__global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs a) {
+__global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs a, uint32_t tiles, uint32_t units) {
     __shared__ __align__(16) uint8_t lds[kLdsBytes];
-    const uint64_t c = blockIdx.y;
-    const int rg = blockIdx.z;
+    // Workgroups are dealt round-robin to the 8 XCDs.  Give every column
+    // tile of one (codeword, row group) unit to the same XCD so the unit's
+    // product table is fetched into one L2 only: the slot-th workgroup on
+    // XCD x takes unit x + 8 * (slot / tiles), tile slot % tiles.
+    const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const uint32_t unit = xcd + kXcds * (slot / tiles);
+    if (unit >= units) return;
+    const uint32_t tile = slot % tiles;
+    const uint64_t c = unit / a.groups_launched;
+    const int rg = static_cast<int>(unit % a.groups_launched);
     const uint32_t R = DEC ? load_uniform(a.rows + c) : a.rmax;
     const uint32_t base = static_cast<uint32_t>(rg) * kRowsPerPass;
     const int rows_here = static_cast<int>(R > base ? min(static_cast<uint32_t>(kRowsPerPass), R - base) : 0u);
     const bool do_copy = DEC && rg == 0;
     if (rows_here == 0 && !do_copy) return;
-    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * kTile + threadIdx.x * kBytesPerThread;
+    const uint64_t p = static_cast<uint64_t>(tile) * kTile + threadIdx.x * kBytesPerThread;
     if (DEC && rows_here == 0) {
         copy_only_pass<VEC>(a, c, p);
     } else if (rows_here <= 4) {
@@ -442,29 +451,34 @@ __global__ void fill_splitmix_kernel(uint8_t *dst, uint64_t len, uint64_t seed, 
 
 }  // namespace
 
-hipError_t launch_rows_terms_lds(const RowsTermsArgs &a, bool vec16, hipStream_t s) {
-    if (a.count == 0 || a.L == 0) return hipSuccess;
+hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
+    if (args.count == 0 || args.L == 0) return hipSuccess;
+    RowsTermsArgs a = args;
     const uint64_t tiles = (a.L + kTile - 1) / kTile;
     const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
-    if (tiles > 0x7fffffffull || a.count > 65535 || groups > 65535 || groups > a.groups + (a.rmax == 0))
-        return hipErrorInvalidValue;
-    dim3 grid(static_cast<uint32_t>(tiles), static_cast<uint32_t>(a.count), groups);
+    if (groups > a.groups + (a.rmax == 0)) return hipErrorInvalidValue;
+    a.groups_launched = groups;
+    const uint64_t units = a.count * groups;
+    const uint64_t blocks = (units + kXcds - 1) / kXcds * kXcds * tiles;
+    if (blocks > 0x7fffffffull || units > 0xffffffffull) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<uint32_t>(blocks));
+    const uint32_t t32 = static_cast<uint32_t>(tiles), u32 = static_cast<uint32_t>(units);
     const bool dec = a.src_ptrs != nullptr;
     if (dec != (a.dst_ptrs != nullptr) || dec != (a.copy_ptrs != nullptr) || dec != (a.rows != nullptr))
         return hipErrorInvalidValue;
     const bool full = vec16 && a.T % kTermChunk == 0;
     if (full && dec)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, true>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (full)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false, true>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (vec16 && dec)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, false>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (vec16)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false, false>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<true, false, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (dec)
-        hipLaunchKernelGGL((rows_terms_lds_kernel<false, true, false>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<false, true, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else
-        hipLaunchKernelGGL((rows_terms_lds_kernel<false, false, false>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((rows_terms_lds_kernel<false, false, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
     return hipGetLastError();
 }
 

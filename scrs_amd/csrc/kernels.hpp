@@ -30,6 +30,7 @@ struct RowsTermsArgs {
     uint32_t T = 0;                   // terms per codeword
     uint32_t rmax = 0;                // rows (upper bound when `rows` is set)
     uint32_t groups = 0;              // 16-row groups per term in the tables
+    uint32_t groups_launched = 0;     // row groups this launch covers (set by the launcher)
     uint64_t L = 0;                   // symbol length in bytes
     uint64_t count = 0;               // codewords
 };
