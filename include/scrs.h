@@ -45,6 +45,9 @@ typedef struct scrs_batch_encoder scrs_batch_encoder;
 typedef struct scrs_batch_decoder scrs_batch_decoder;
 typedef struct scrs_encode_scratch scrs_encode_scratch;
 typedef struct scrs_decode_scratch scrs_decode_scratch;
+typedef struct scrs_decoder scrs_decoder;
+typedef struct scrs_decoder_scratch scrs_decoder_scratch;
+typedef struct scrs_incr_encoder scrs_incr_encoder;
 
 /* ---- Library ---------------------------------------------------------- */
 
@@ -147,6 +150,106 @@ scrs_status scrs_decode_batch_dev(scrs_batch_decoder *dec, const uint16_t *indic
 scrs_status scrs_decode_batch_ptrs_dev(scrs_batch_decoder *dec, const uint16_t *indices,
                                        const uint8_t *const *payload_ptrs_dev, uint8_t *out_dev,
                                        uint64_t count, void *stream);
+
+/* ---- Decoder: streaming receive (codec.rs:180-201) ---------------------- */
+
+/* decoder(&profile) (selector.rs:117-125).  Device buffers for n received
+ * symbols are reserved up front; allocation failure is TooManySymbols{cap}
+ * (tower/decoder.rs:98-108). */
+scrs_status scrs_decoder_create(const scrs_profile *profile, scrs_decoder **out);
+void scrs_decoder_destroy(scrs_decoder *dec);
+scrs_status scrs_decoder_profile(const scrs_decoder *dec, scrs_profile *out);
+
+/* SymbolSink::push (transport/symbol_sink.rs:36-54; decoder/streaming.rs:387-429;
+ * tower/decoder.rs:371-405).  Check order: IndexOutOfRange{index,n} ->
+ * WrongPayloadLen -> TooManySymbols{cap=n, received}; then received += 1;
+ * a duplicate index or a push after completion is Dependent; otherwise the
+ * payload is stored and the outcome is Complete or Advanced{rank, received}.
+ * The payload is copied before the call returns. */
+scrs_status scrs_decoder_push(scrs_decoder *dec, uint64_t index, const uint8_t *payload,
+                              uint64_t len, scrs_push_outcome *out);
+
+/* push with a device-resident payload, copied on `stream` (no reference
+ * counterpart; same checks and outcomes as scrs_decoder_push). */
+scrs_status scrs_decoder_push_dev(scrs_decoder *dec, uint64_t index, const uint8_t *payload_dev,
+                                  uint64_t len, void *stream, scrs_push_outcome *out);
+
+/* Decoder::{rank, received} (codec.rs:187-189), SymbolSink::is_complete. */
+uint64_t scrs_decoder_rank(const scrs_decoder *dec);
+uint64_t scrs_decoder_received(const scrs_decoder *dec);
+int32_t scrs_decoder_is_complete(const scrs_decoder *dec);
+
+/* Decoder::reset (codec.rs:191): clears the receipt state, keeps buffers. */
+scrs_status scrs_decoder_reset(scrs_decoder *dec);
+
+/* Decoder::scratch (codec.rs:185; selector.rs:188-196): a 16-entry recipe
+ * cache for the GF(256) engines (decoder/streaming.rs:461-463,
+ * decoder/cache.rs), a geometry-tagged workspace for Tower. */
+scrs_status scrs_decoder_scratch_create(const scrs_decoder *dec, scrs_decoder_scratch **out);
+void scrs_decoder_scratch_destroy(scrs_decoder_scratch *scratch);
+/* RecipeCache::{hits, misses, len} and the bytes of the cached recipes
+ * (decoder/cache.rs:60-100). */
+scrs_status scrs_decoder_scratch_stats(const scrs_decoder_scratch *scratch, uint64_t *hits,
+                                       uint64_t *misses, uint64_t *len, uint64_t *recipe_bytes);
+
+/* Decoder::finalize_into (codec.rs:194; decoder/streaming.rs:476-488).
+ * Check order: InsufficientRank{rank,k} -> WrongOutputLen. */
+scrs_status scrs_decoder_finalize_into(scrs_decoder *dec, uint8_t *out, uint64_t out_len);
+
+/* Decoder::finalize_into_with (codec.rs:196-200; selector.rs:224-240).  A
+ * scratch of another engine kind is ScratchMismatch (checked first); then
+ * InsufficientRank -> WrongOutputLen -> ScratchMismatch for a Tower scratch
+ * of another geometry (tower/decoder.rs:200-221). */
+scrs_status scrs_decoder_finalize_into_with(scrs_decoder *dec, uint8_t *out, uint64_t out_len,
+                                            scrs_decoder_scratch *scratch);
+
+/* finalize_into writing device memory on `stream` (no reference
+ * counterpart; same checks as scrs_decoder_finalize_into). */
+scrs_status scrs_decoder_finalize_into_dev(scrs_decoder *dec, uint8_t *out_dev, uint64_t out_len,
+                                           void *stream);
+
+/* ---- IncrementalEncoder (codec.rs:122-131) ------------------------------ */
+
+/* incremental_encoder(&profile) (selector.rs:370-381): Good Cauchy and
+ * Tower; any other engine is UnsupportedMode{engine}. */
+scrs_status scrs_incr_encoder_create(const scrs_profile *profile, scrs_incr_encoder **out);
+void scrs_incr_encoder_destroy(scrs_incr_encoder *enc);
+scrs_status scrs_incr_encoder_profile(const scrs_incr_encoder *enc, scrs_profile *out);
+
+/* IncrementalEncoder::feed (encoder/streaming.rs:162-190; tower/encoder.rs:89-116).
+ * Check order: IndexOutOfRange{index, n=k+m} for index >= k ->
+ * WrongPayloadLen -> DuplicateData{index}.  The data is copied before the
+ * call returns; contributions are folded in batches on the device. */
+scrs_status scrs_incr_feed(scrs_incr_encoder *enc, uint64_t index, const uint8_t *data,
+                           uint64_t len);
+
+/* feed with device-resident data read on `stream` (no reference counterpart;
+ * same checks as scrs_incr_feed). */
+scrs_status scrs_incr_feed_dev(scrs_incr_encoder *enc, uint64_t index, const uint8_t *data_dev,
+                               uint64_t len, void *stream);
+
+/* IncrementalEncoder::repair (encoder/streaming.rs:197-210; tower/encoder.rs:122-131):
+ * IndexOutOfRange{index, n} for index >= m.  *ptr receives a host view of
+ * repair `index` (the partial sum of the data fed so far), valid until the
+ * next feed, reset or destroy. */
+scrs_status scrs_incr_repair(scrs_incr_encoder *enc, uint64_t index, const uint8_t **ptr,
+                             uint64_t *len);
+
+/* Device view of all m repair accumulators (m * symbol_len bytes, repair j
+ * at j * symbol_len), current for work enqueued on `stream` after the call
+ * (no reference counterpart). */
+scrs_status scrs_incr_repairs_dev(scrs_incr_encoder *enc, uint8_t **repairs_dev, void *stream);
+
+/* IncrementalEncoder::{fed_count, reset} (codec.rs:128-130). */
+uint64_t scrs_incr_fed_count(const scrs_incr_encoder *enc);
+scrs_status scrs_incr_reset(scrs_incr_encoder *enc);
+
+/* Batched device encode with the incremental encoder's coefficients, equal
+ * to feeding all k data symbols of each of `count` codewords (no reference
+ * counterpart).  Layout as scrs_encode_batch_dev.  The only batched encoder
+ * of the Tower engine. */
+scrs_status scrs_incr_encode_batch_dev(const scrs_incr_encoder *enc, const uint8_t *data_dev,
+                                       uint8_t *repairs_dev, uint64_t count, void *stream);
 
 /* ---- Utilities (no reference counterpart) ----------------------------- */
 

@@ -7,6 +7,10 @@ reference's ``Profile`` selection and codec traits.
 from .codec import (  # noqa: F401
     BatchDecoder,
     BatchEncoder,
+    Decoder,
+    DecoderScratch,
+    IncrementalEncoder,
+    PushOutcome,
     ConfigError,
     DecodeError,
     DeviceError,
@@ -17,6 +21,8 @@ from .codec import (  # noqa: F401
     ScrsError,
     batch_decoder,
     batch_encoder,
+    decoder,
+    incremental_encoder,
     fill_random_dev,
     recommended_gf16_engine,
 )

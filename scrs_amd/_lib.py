@@ -57,6 +57,33 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_decode_into_with": (Status, [H, u64p, C.POINTER(vp), u64p, C.c_uint64, vp, C.c_uint64, H]),
     "scrs_decode_batch_dev": (Status, [H, u16p, vp, vp, vp, C.c_uint64, vp]),
     "scrs_decode_batch_ptrs_dev": (Status, [H, u16p, vp, vp, C.c_uint64, vp]),
+    # streaming decoder
+    "scrs_decoder_create": (Status, [C.POINTER(ProfileStruct), HP]),
+    "scrs_decoder_destroy": (None, [H]),
+    "scrs_decoder_profile": (Status, [H, C.POINTER(ProfileStruct)]),
+    "scrs_decoder_push": (Status, [H, C.c_uint64, vp, C.c_uint64, C.POINTER(PushOutcomeStruct)]),
+    "scrs_decoder_push_dev": (Status, [H, C.c_uint64, vp, C.c_uint64, vp, C.POINTER(PushOutcomeStruct)]),
+    "scrs_decoder_rank": (C.c_uint64, [H]),
+    "scrs_decoder_received": (C.c_uint64, [H]),
+    "scrs_decoder_is_complete": (C.c_int32, [H]),
+    "scrs_decoder_reset": (Status, [H]),
+    "scrs_decoder_scratch_create": (Status, [H, HP]),
+    "scrs_decoder_scratch_destroy": (None, [H]),
+    "scrs_decoder_scratch_stats": (Status, [H, u64p, u64p, u64p, u64p]),
+    "scrs_decoder_finalize_into": (Status, [H, vp, C.c_uint64]),
+    "scrs_decoder_finalize_into_with": (Status, [H, vp, C.c_uint64, H]),
+    "scrs_decoder_finalize_into_dev": (Status, [H, vp, C.c_uint64, vp]),
+    # incremental encoder
+    "scrs_incr_encoder_create": (Status, [C.POINTER(ProfileStruct), HP]),
+    "scrs_incr_encoder_destroy": (None, [H]),
+    "scrs_incr_encoder_profile": (Status, [H, C.POINTER(ProfileStruct)]),
+    "scrs_incr_feed": (Status, [H, C.c_uint64, vp, C.c_uint64]),
+    "scrs_incr_feed_dev": (Status, [H, C.c_uint64, vp, C.c_uint64, vp]),
+    "scrs_incr_repair": (Status, [H, C.c_uint64, C.POINTER(vp), u64p]),
+    "scrs_incr_repairs_dev": (Status, [H, C.POINTER(vp), vp]),
+    "scrs_incr_fed_count": (C.c_uint64, [H]),
+    "scrs_incr_reset": (Status, [H]),
+    "scrs_incr_encode_batch_dev": (Status, [H, vp, vp, C.c_uint64, vp]),
     # utilities
     "scrs_fill_random_dev": (Status, [vp, C.c_uint64, C.c_uint64, C.c_uint64, vp]),
 }

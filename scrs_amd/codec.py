@@ -382,6 +382,188 @@ def batch_decoder(profile: Profile) -> BatchDecoder:
     return BatchDecoder(profile)
 
 
+# ---------------------------------------------------------------------------
+# Decoder (streaming)
+# ---------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class PushOutcome:
+    """PushOutcome (transport/symbol_sink.rs:8-28): ``Advanced`` carries the
+    rank and received count, ``Complete`` and ``Dependent`` carry nothing."""
+
+    kind: str
+    rank: int | None = None
+    received: int | None = None
+
+    @staticmethod
+    def advanced(rank: int, received: int) -> "PushOutcome":
+        return PushOutcome("Advanced", rank, received)
+
+
+PushOutcome.COMPLETE = PushOutcome("Complete")
+PushOutcome.DEPENDENT = PushOutcome("Dependent")
+_PUSH_KINDS = {0: "Advanced", 1: "Complete", 2: "Dependent"}
+
+
+def _push_outcome(o: _lib.PushOutcomeStruct) -> PushOutcome:
+    kind = _PUSH_KINDS[o.kind]
+    if kind == "Advanced":
+        return PushOutcome.advanced(int(o.rank), int(o.received))
+    return PushOutcome(kind)
+
+
+class DecoderScratch(_Handle):
+    """Decoder::Scratch: a recipe cache (GF(2^8)) or a geometry-tagged
+    workspace (Tower)."""
+
+    _destroy_name = "scrs_decoder_scratch_destroy"
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        check(_lib.load().scrs_decoder_scratch_stats(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("hits", "misses", "len", "recipe_bytes"), (int(x.value) for x in v)))
+
+
+class Decoder(_Coded):
+    """Decoder + SymbolSink (codec.rs:180-201, transport/symbol_sink.rs:36-54)
+    on the device: pushes land in device slots, finalize reconstructs there."""
+
+    _destroy_name = "scrs_decoder_destroy"
+
+    def __init__(self, profile: Profile):
+        h = C.c_void_p()
+        check(_lib.load().scrs_decoder_create(C.byref(profile._struct()), C.byref(h)))
+        super().__init__(h.value)
+        self.profile = profile
+
+    def scratch(self) -> DecoderScratch:
+        h = C.c_void_p()
+        check(_lib.load().scrs_decoder_scratch_create(self._h, C.byref(h)))
+        return DecoderScratch(h.value)
+
+    def push(self, index: int, payload) -> PushOutcome:
+        if index < 0:
+            raise DecodeError("IndexOutOfRange", index=index, n=self.n)
+        p, n, _k = _in_buf(payload)
+        o = _lib.PushOutcomeStruct()
+        check(_lib.load().scrs_decoder_push(self._h, index, p, n, C.byref(o)))
+        return _push_outcome(o)
+
+    def push_dev(self, index: int, payload_dev, length: int, stream=None) -> PushOutcome:
+        o = _lib.PushOutcomeStruct()
+        check(_lib.load().scrs_decoder_push_dev(self._h, index, _dev_ptr(payload_dev), length,
+                                                _stream_ptr(stream), C.byref(o)))
+        return _push_outcome(o)
+
+    def is_complete(self) -> bool:
+        return bool(_lib.load().scrs_decoder_is_complete(self._h))
+
+    def rank(self) -> int:
+        return int(_lib.load().scrs_decoder_rank(self._h))
+
+    def received(self) -> int:
+        return int(_lib.load().scrs_decoder_received(self._h))
+
+    def reset(self) -> None:
+        check(_lib.load().scrs_decoder_reset(self._h))
+
+    def finalize_into(self, out) -> None:
+        op, on, _k = _out_buf(out)
+        check(_lib.load().scrs_decoder_finalize_into(self._h, op, on))
+
+    def finalize_into_with(self, out, scratch: DecoderScratch) -> None:
+        op, on, _k = _out_buf(out)
+        check(_lib.load().scrs_decoder_finalize_into_with(self._h, op, on, scratch._h))
+
+    def finalize_into_dev(self, out_dev, length: int, stream=None) -> None:
+        check(_lib.load().scrs_decoder_finalize_into_dev(self._h, _dev_ptr(out_dev), length, _stream_ptr(stream)))
+
+    def finalize(self) -> np.ndarray:
+        """SymbolSink::finalize (returns the data; the decoder stays usable)."""
+        out = np.empty(self.k * self.symbol_len, np.uint8)
+        self.finalize_into(out)
+        return out
+
+    # BatchDecoder for every Decoder (codec.rs:203-240): WrongCount, then
+    # reset, push each symbol (a Dependent push is DuplicateIndex), finalize.
+    def decode_into_with(self, symbols: Sequence[tuple[int, object]], out, scratch: DecoderScratch) -> None:
+        if len(symbols) != self.k:
+            raise DecodeError("WrongCount", expected=self.k, got=len(symbols))
+        self.reset()
+        for index, payload in symbols:
+            if self.push(index, payload) == PushOutcome.DEPENDENT:
+                raise DecodeError("DuplicateIndex", index=index)
+        self.finalize_into_with(out, scratch)
+
+    def decode_into(self, symbols: Sequence[tuple[int, object]], out) -> None:
+        with self.scratch() as scratch:
+            self.decode_into_with(symbols, out, scratch)
+
+
+def decoder(profile: Profile) -> Decoder:
+    """decoder (selector.rs:117-125)."""
+    return Decoder(profile)
+
+
+# ---------------------------------------------------------------------------
+# IncrementalEncoder
+# ---------------------------------------------------------------------------
+
+
+class IncrementalEncoder(_Coded):
+    """IncrementalEncoder (codec.rs:122-131) on the device: fed symbols are
+    folded into device-resident repair accumulators."""
+
+    _destroy_name = "scrs_incr_encoder_destroy"
+
+    def __init__(self, profile: Profile):
+        h = C.c_void_p()
+        check(_lib.load().scrs_incr_encoder_create(C.byref(profile._struct()), C.byref(h)))
+        super().__init__(h.value)
+        self.profile = profile
+
+    def feed(self, index: int, data) -> None:
+        if index < 0:
+            raise EncodeError("IndexOutOfRange", index=index, n=self.n)
+        p, n, _k = _in_buf(data)
+        check(_lib.load().scrs_incr_feed(self._h, index, p, n))
+
+    def feed_dev(self, index: int, data_dev, length: int, stream=None) -> None:
+        check(_lib.load().scrs_incr_feed_dev(self._h, index, _dev_ptr(data_dev), length, _stream_ptr(stream)))
+
+    def repair(self, index: int) -> bytes:
+        """Repair ``index`` as the partial sum of the data fed so far (a copy
+        of the library's view)."""
+        if index < 0:
+            raise EncodeError("IndexOutOfRange", index=index, n=self.n)
+        ptr = C.c_void_p()
+        ln = C.c_uint64()
+        check(_lib.load().scrs_incr_repair(self._h, index, C.byref(ptr), C.byref(ln)))
+        return C.string_at(ptr.value, ln.value)
+
+    def repairs_dev(self, stream=None) -> int:
+        """Device address of the m x symbol_len repair accumulators."""
+        ptr = C.c_void_p()
+        check(_lib.load().scrs_incr_repairs_dev(self._h, C.byref(ptr), _stream_ptr(stream)))
+        return int(ptr.value)
+
+    def fed_count(self) -> int:
+        return int(_lib.load().scrs_incr_fed_count(self._h))
+
+    def reset(self) -> None:
+        check(_lib.load().scrs_incr_reset(self._h))
+
+    def encode_batch_dev(self, data_dev, repairs_dev, count: int, stream=None) -> None:
+        check(_lib.load().scrs_incr_encode_batch_dev(self._h, _dev_ptr(data_dev), _dev_ptr(repairs_dev), count,
+                                                     _stream_ptr(stream)))
+
+
+def incremental_encoder(profile: Profile) -> IncrementalEncoder:
+    """incremental_encoder (selector.rs:370-381)."""
+    return IncrementalEncoder(profile)
+
+
 def fill_random_dev(dst_dev, length: int, seed: int, byte_offset: int = 0, stream=None) -> None:
     """Seeded splitmix64 counter-stream bytes written on the device."""
     check(_lib.load().scrs_fill_random_dev(_dev_ptr(dst_dev), length, seed, byte_offset, _stream_ptr(stream)))

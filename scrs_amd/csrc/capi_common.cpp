@@ -1,42 +1,6 @@
-// capi_gf16.cpp - GF(2^16) engines (Tower, AFFT) behind the shared handles.
-//
-// The GF(2^16) device engines are not built yet: every entry point reports
-// UnsupportedMode for the requested engine rather than computing anything.
+// capi_common.cpp - status names and the additive-FFT hooks.
+#include "handles.hpp"
 #include "status.hpp"
-
-scrs_status scrs_afft_encoder_create_impl(const scrs_profile *p, scrs_batch_encoder **out) {
-    *out = nullptr;
-    return st(SCRS_CONFIG_UNSUPPORTED_MODE, static_cast<uint64_t>(p->engine));
-}
-
-void scrs_afft_encoder_destroy_impl(scrs_batch_encoder *) {}
-
-scrs_status scrs_afft_encode_into_impl(const scrs_batch_encoder *, const uint8_t *, uint64_t, uint8_t *, uint64_t,
-                                       scrs_encode_scratch *) {
-    return st(SCRS_CONFIG_UNSUPPORTED_MODE, SCRS_ENGINE_AFFT);
-}
-
-scrs_status scrs_afft_encode_batch_dev_impl(const scrs_batch_encoder *, const uint8_t *, uint8_t *, uint64_t,
-                                            void *) {
-    return st(SCRS_CONFIG_UNSUPPORTED_MODE, SCRS_ENGINE_AFFT);
-}
-
-scrs_status scrs_gf16_batch_decoder_create_impl(const scrs_profile *p, scrs_batch_decoder **out) {
-    *out = nullptr;
-    return st(SCRS_CONFIG_UNSUPPORTED_MODE, static_cast<uint64_t>(p->engine));
-}
-
-void scrs_gf16_batch_decoder_destroy_impl(scrs_batch_decoder *) {}
-
-scrs_status scrs_gf16_decode_into_impl(scrs_batch_decoder *, const uint64_t *, const uint8_t *const *,
-                                       const uint64_t *, uint64_t, uint8_t *, uint64_t, scrs_decode_scratch *) {
-    return st(SCRS_CONFIG_UNSUPPORTED_MODE, SCRS_ENGINE_TOWER);
-}
-
-scrs_status scrs_gf16_decode_batch_impl(scrs_batch_decoder *, const uint16_t *, const uint8_t *const *,
-                                        const uint8_t *, const uint8_t *, uint8_t *, uint64_t, void *) {
-    return st(SCRS_CONFIG_UNSUPPORTED_MODE, SCRS_ENGINE_TOWER);
-}
 
 extern "C" const char *scrs_status_name(int32_t code) {
     switch (code) {

@@ -1,0 +1,233 @@
+// engine.cpp - device decode engine and coefficient plans.
+#include "engine.hpp"
+
+#include <cstring>
+
+#include "matrices.hpp"
+#include "status.hpp"
+
+namespace scrs {
+
+size_t engine_capacity(int32_t engine) {
+    switch (engine) {
+        case SCRS_ENGINE_STANDARD_CAUCHY: return kCapStandard;
+        case SCRS_ENGINE_GOOD_CAUCHY: return kCapGood;
+        case SCRS_ENGINE_TOWER: return kCapTower;
+        default: return kCapAfft;
+    }
+}
+
+std::vector<uint16_t> cauchy_coordinates(const scrs_profile &p) {
+    const size_t n = p.k + p.m;
+    std::vector<uint16_t> c(n);
+    if (p.engine == SCRS_ENGINE_STANDARD_CAUCHY) {
+        for (size_t i = 0; i < n; i++) c[i] = static_cast<uint16_t>(i);
+    } else if (p.engine == SCRS_ENGINE_GOOD_CAUCHY) {
+        for (size_t i = 0; i < n; i++) c[i] = gf256().pow3(i);
+    } else {
+        uint16_t v = 1;
+        for (size_t i = 0; i < n; i++) {
+            c[i] = v;
+            v = Gf16::mul(v, Gf16::kGenerator);
+        }
+    }
+    return c;
+}
+
+std::vector<uint32_t> coefficient_tables(const scrs_profile &p, const std::vector<uint32_t> &terms,
+                                         uint32_t *groups_out) {
+    const size_t k = p.k, m = p.m, T = terms.size();
+    const uint32_t groups = static_cast<uint32_t>((m + 15) / 16);
+    *groups_out = groups;
+    if (engine_is_gf256(p.engine)) {
+        const std::vector<uint8_t> c = p.engine == SCRS_ENGINE_GOOD_CAUCHY ? good_cauchy_matrix(k, m)
+                                                                           : standard_cauchy_matrix(k, m);
+        std::vector<uint8_t> coef(T * m);
+        for (size_t t = 0; t < T; t++)
+            for (size_t j = 0; j < m; j++) coef[t * m + j] = c[terms[t] * m + j];
+        std::vector<uint32_t> tab(T * groups * 128);
+        build_nibble_tables(coef.data(), T, m, groups, tab.data());
+        return tab;
+    }
+    const std::vector<uint16_t> c = tower_cauchy_matrix(k, m);
+    std::vector<uint16_t> coef(T * m);
+    for (size_t t = 0; t < T; t++)
+        for (size_t j = 0; j < m; j++) coef[t * m + j] = c[terms[t] * m + j];
+    std::vector<uint32_t> tab(T * groups * 512);
+    build_nibble_tables16(coef.data(), T, m, groups, tab.data());
+    return tab;
+}
+
+hipError_t launch_rows_terms_field(bool gf16, const RowsTermsArgs &a, bool vec16, hipStream_t s) {
+    return gf16 ? launch_rows_terms16(a, vec16, s) : launch_rows_terms_lds(a, vec16, s);
+}
+
+DecodeEngine::~DecodeEngine() {
+    if (idx_copied) (void)hipEventDestroy(idx_copied);
+}
+
+scrs_status DecodeEngine::init(const scrs_profile &p) {
+    prof = p;
+    gf16 = !engine_is_gf256(p.engine);
+    k = static_cast<uint32_t>(p.k);
+    m = static_cast<uint32_t>(p.m);
+    L = p.symbol_len;
+    rmax = k < m ? k : m;
+    groups = (rmax + 15) / 16;
+    const std::vector<uint16_t> coords = cauchy_coordinates(p);
+    hipError_t err = d_coords.ensure(coords.size() * sizeof(uint16_t));
+    if (err == hipSuccess)
+        err = hipMemcpy(d_coords.ptr, coords.data(), coords.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&idx_copied, hipEventDisableTiming);
+    if (err == hipSuccess) err = d_flag.ensure(sizeof(uint32_t));
+    if (err == hipSuccess) err = hipMemset(d_flag.ptr, 0, sizeof(uint32_t));
+    if (err != hipSuccess) {
+        if (err == hipErrorOutOfMemory) return st(SCRS_CONFIG_TOO_MANY_SYMBOLS, engine_capacity(p.engine));
+        return st(SCRS_ERR_DEVICE, err);
+    }
+    // Bound the per-launch workspace: product tables dominate.
+    const uint64_t table_bytes = static_cast<uint64_t>(k) * groups * table_group_bytes(gf16);
+    const uint64_t budget = 1ull << 30;
+    chunk = table_bytes ? budget / table_bytes : 65535;
+    if (chunk > 65535) chunk = 65535;
+    return ok();
+}
+
+scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs,
+                                  const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out,
+                                  uint64_t count, hipStream_t s, bool vec16) {
+    if (count == 0) return ok();
+    if (chunk == 0) return st(SCRS_ERR_CAPACITY, static_cast<uint64_t>(k) * groups * table_group_bytes(gf16));
+    const uint64_t want = count < chunk ? count : chunk;
+    hipError_t err;
+    if (want > cap) {
+        const uint64_t kk = k, rr = rmax ? rmax : 1;
+        const uint64_t work = cauchy_solve_work_elems(k, rmax);
+        if ((err = d_idx.ensure(want * kk * 2)) != hipSuccess || (err = h_idx.ensure(want * kk * 2)) != hipSuccess ||
+            (err = d_fused.ensure(want * rr * kk * 2)) != hipSuccess ||
+            (err = d_missing.ensure(want * rr * 2)) != hipSuccess || (err = d_rows.ensure(want * 4)) != hipSuccess ||
+            (err = d_work.ensure(want * work * 2)) != hipSuccess ||
+            (err = d_tables.ensure(want * kk * groups * table_group_bytes(gf16))) != hipSuccess ||
+            (err = d_src.ensure(want * kk * 8)) != hipSuccess || (err = d_dst.ensure(want * rr * 8)) != hipSuccess ||
+            (err = d_copy.ensure(want * kk * 8)) != hipSuccess)
+            return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+        cap = want;
+    }
+    for (uint64_t c0 = 0; c0 < count; c0 += cap) {
+        const uint64_t cn = count - c0 < cap ? count - c0 : cap;
+        // The pinned index buffer is reused: wait for the previous upload.
+        if ((err = hipEventSynchronize(idx_copied)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        std::memcpy(h_idx.ptr, indices + c0 * k, cn * k * sizeof(uint16_t));
+        if ((err = hipMemcpyAsync(d_idx.ptr, h_idx.ptr, cn * k * sizeof(uint16_t), hipMemcpyHostToDevice, s)) !=
+                hipSuccess ||
+            (err = hipEventRecord(idx_copied, s)) != hipSuccess)
+            return st(SCRS_ERR_DEVICE, err);
+        CauchySolveArgs sa;
+        sa.indices = d_idx.as<uint16_t>();
+        sa.coords = d_coords.as<uint16_t>();
+        sa.fused = d_fused.as<uint16_t>();
+        sa.missing = d_missing.as<uint16_t>();
+        sa.rows = d_rows.as<uint32_t>();
+        sa.work = d_work.as<uint16_t>();
+        sa.work_stride = cauchy_solve_work_elems(k, rmax);
+        sa.error_flag = d_flag.as<uint32_t>();
+        sa.k = k;
+        sa.m = m;
+        sa.rmax = rmax;
+        sa.count = cn;
+        if ((err = launch_cauchy_solve(sa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        DecodePrepArgs pa;
+        pa.indices = d_idx.as<uint16_t>();
+        pa.fused = d_fused.as<uint16_t>();
+        pa.missing = d_missing.as<uint16_t>();
+        pa.rows = d_rows.as<uint32_t>();
+        pa.payload_ptrs = payload_ptrs ? payload_ptrs + c0 * k : nullptr;
+        pa.data_base = data_base ? data_base + c0 * k * L : nullptr;
+        pa.repair_base = repair_base ? repair_base + c0 * m * L : nullptr;
+        pa.out_base = out + c0 * k * L;
+        pa.tables = d_tables.as<uint32_t>();
+        pa.src_ptrs = d_src.as<const uint8_t *>();
+        pa.dst_ptrs = d_dst.as<uint8_t *>();
+        pa.copy_ptrs = d_copy.as<uint8_t *>();
+        pa.k = k;
+        pa.m = m;
+        pa.rmax = rmax;
+        pa.groups = groups;
+        pa.L = L;
+        pa.count = cn;
+        if ((err = launch_decode_prepare(pa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        RowsTermsArgs ra;
+        ra.src_ptrs = d_src.as<const uint8_t *const>();
+        ra.dst_ptrs = d_dst.as<uint8_t *const>();
+        ra.copy_ptrs = d_copy.as<uint8_t *const>();
+        ra.rows = d_rows.as<uint32_t>();
+        ra.tables = d_tables.as<uint32_t>();
+        ra.table_cw_stride = static_cast<uint64_t>(k) * groups * table_group_bytes(gf16) / 4;
+        ra.T = k;
+        ra.rmax = rmax;
+        ra.groups = groups;
+        ra.L = L;
+        ra.count = cn;
+        if ((err = launch_rows_terms_field(gf16, ra, vec16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    }
+    return ok();
+}
+
+scrs_status DecodeEngine::enqueue_recipe(const uint16_t *indices, const uint32_t *tables, const uint16_t *missing,
+                                         const uint32_t *rows, const uint8_t *data_base,
+                                         const uint8_t *repair_base, uint8_t *out, hipStream_t s, bool vec16) {
+    hipError_t err;
+    if (cap == 0) {
+        // Pointer tables and indices for one codeword.
+        if ((err = d_idx.ensure(k * 2ull)) != hipSuccess || (err = h_idx.ensure(k * 2ull)) != hipSuccess ||
+            (err = d_src.ensure(k * 8ull)) != hipSuccess || (err = d_dst.ensure((rmax ? rmax : 1) * 8ull)) != hipSuccess ||
+            (err = d_copy.ensure(k * 8ull)) != hipSuccess)
+            return st(SCRS_ERR_DEVICE, err);
+    }
+    if ((err = hipEventSynchronize(idx_copied)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    std::memcpy(h_idx.ptr, indices, k * sizeof(uint16_t));
+    if ((err = hipMemcpyAsync(d_idx.ptr, h_idx.ptr, k * sizeof(uint16_t), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (err = hipEventRecord(idx_copied, s)) != hipSuccess)
+        return st(SCRS_ERR_DEVICE, err);
+    DecodePrepArgs pa;
+    pa.indices = d_idx.as<uint16_t>();
+    pa.missing = missing;
+    pa.rows = rows;
+    pa.data_base = data_base;
+    pa.repair_base = repair_base;
+    pa.out_base = out;
+    pa.src_ptrs = d_src.as<const uint8_t *>();
+    pa.dst_ptrs = d_dst.as<uint8_t *>();
+    pa.copy_ptrs = d_copy.as<uint8_t *>();
+    pa.k = k;
+    pa.m = m;
+    pa.rmax = rmax;
+    pa.groups = groups;
+    pa.L = L;
+    pa.count = 1;
+    if ((err = launch_decode_prepare(pa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    RowsTermsArgs ra;
+    ra.src_ptrs = d_src.as<const uint8_t *const>();
+    ra.dst_ptrs = d_dst.as<uint8_t *const>();
+    ra.copy_ptrs = d_copy.as<uint8_t *const>();
+    ra.rows = rows;
+    ra.tables = tables;
+    ra.table_cw_stride = 0;
+    ra.T = k;
+    ra.rmax = rmax;
+    ra.groups = groups;
+    ra.L = L;
+    ra.count = 1;
+    err = launch_rows_terms_field(gf16, ra, vec16, s);
+    return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
+}
+
+hipError_t DecodeEngine::read_error(uint32_t *flag, hipStream_t s) {
+    hipError_t err = hipMemcpyAsync(flag, d_flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (err != hipSuccess) return err;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t DecodeEngine::clear_error(hipStream_t s) { return hipMemsetAsync(d_flag.ptr, 0, sizeof(uint32_t), s); }
+
+}  // namespace scrs

@@ -1,0 +1,78 @@
+// engine.hpp - device decode engine and coefficient plans shared by the
+// batch decoder, the streaming decoder and the incremental encoder.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/scrs.h"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace scrs {
+
+inline bool engine_is_gf256(int32_t engine) {
+    return engine == SCRS_ENGINE_STANDARD_CAUCHY || engine == SCRS_ENGINE_GOOD_CAUCHY;
+}
+
+size_t engine_capacity(int32_t engine);
+
+// Field coordinates of every codeword index: data i -> x_i, repair j -> y_j.
+// Standard Cauchy x_i = i, y_j = k + j (matrices/cauchy.rs:88-99); Good
+// Cauchy 3^i, 3^(k+j) (matrices/good_cauchy.rs:59-64); Tower g^i, g^(k+j)
+// with g = 0x0108 (tower/cauchy.rs:46-63).
+std::vector<uint16_t> cauchy_coordinates(const scrs_profile &p);
+
+// Product-table bytes per (term, 16-row group).
+inline uint64_t table_group_bytes(bool gf16) { return gf16 ? 2048 : 512; }
+
+// k x m coefficient block of a Cauchy engine as a table for `T` terms, all
+// m rows, one group layout per field.  `terms` lists the source rows (data
+// indices) in term order.
+std::vector<uint32_t> coefficient_tables(const scrs_profile &p, const std::vector<uint32_t> &terms,
+                                         uint32_t *groups_out);
+
+// Launch the multiply-XOR of the profile's field.
+hipError_t launch_rows_terms_field(bool gf16, const RowsTermsArgs &a, bool vec16, hipStream_t s);
+
+// Solve + tables + pointers + multiply-XOR for batches of codewords of one
+// Cauchy geometry (either field).
+struct DecodeEngine {
+    scrs_profile prof{};
+    bool gf16 = false;
+    uint32_t k = 0, m = 0, rmax = 0, groups = 0;
+    uint64_t L = 0;
+    DevBuf d_coords;
+    DevBuf d_idx, d_fused, d_missing, d_rows, d_work, d_flag, d_tables, d_src, d_dst, d_copy;
+    HostBuf h_idx;
+    hipEvent_t idx_copied = nullptr;
+    uint64_t cap = 0;     // codewords the workspace holds
+    uint64_t chunk = 0;   // codewords per launch (bounded table memory)
+
+    DecodeEngine() = default;
+    DecodeEngine(const DecodeEngine &) = delete;
+    DecodeEngine &operator=(const DecodeEngine &) = delete;
+    ~DecodeEngine();
+
+    // Returns a construction status (TooManySymbols{cap} on device OOM).
+    scrs_status init(const scrs_profile &p);
+    // Enqueue the decode of `count` codewords whose indices (host memory,
+    // count x k, received order) are already validated.  Received payloads
+    // come from `payload_ptrs` (device array, count x k) or from the
+    // back-to-back data / repair layout.  Output rows: out + c*k*L + i*L.
+    scrs_status enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
+                        const uint8_t *repair_base, uint8_t *out, uint64_t count, hipStream_t s, bool vec16);
+    // Decode one codeword with a reconstruction recipe computed earlier (its
+    // product tables, missing rows and row count on the device): only the
+    // pointer tables and the multiply-XOR run.
+    scrs_status enqueue_recipe(const uint16_t *indices, const uint32_t *tables, const uint16_t *missing,
+                               const uint32_t *rows, const uint8_t *data_base, const uint8_t *repair_base,
+                               uint8_t *out, hipStream_t s, bool vec16);
+    // Bytes of one codeword's recipe: tables, missing rows, row count.
+    uint64_t recipe_table_bytes() const { return static_cast<uint64_t>(k) * groups * table_group_bytes(gf16); }
+    // Non-zero if a solve found an inconsistent submatrix since the last reset.
+    hipError_t read_error(uint32_t *flag, hipStream_t s);
+    hipError_t clear_error(hipStream_t s);
+};
+
+}  // namespace scrs

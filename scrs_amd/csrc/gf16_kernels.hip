@@ -1,0 +1,240 @@
+// gf16_kernels.hip - GF(2^16) Tower multiply-XOR for gfx950.
+//
+// Symbols are interleaved little-endian u16 field elements
+// (scrs/src/algebra/gf65536.rs:53-60).  The multiply-XOR
+//   dst_r ^= sum_t C[t][r] * src_t
+// is the reference's tower AXPY family (scrs/src/tower/payload.rs:7,158,433)
+// over a batch of codewords.  Multiplication by a constant is linear over
+// GF(2), so c * x = sum over the four nibbles q of x of c * (n_q << 4q).  The
+// product table of term t and 16-row group g holds, for every nibble
+// position q and value n, the 16 row products (2 bytes each): one element
+// costs four 32-byte LDS lookups and eight XOR3s for all 16 rows.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace scrs {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTile = kThreads * 16;          // bytes per workgroup tile
+constexpr int kTermChunk = 16;                // terms staged per LDS fill
+constexpr int kRowsPerPass = 16;
+constexpr int kGroupBytes = 4 * 16 * 32;      // one (term, group) table: 2 KiB
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+typedef __attribute__((address_space(4))) const uint64_t *const_u64_ptr;
+typedef __attribute__((address_space(4))) const uint32_t *const_u32_ptr;
+template <typename T>
+__device__ __forceinline__ T *load_uniform(T *const *ptr) {
+    return reinterpret_cast<T *>(*(const_u64_ptr)(ptr));
+}
+__device__ __forceinline__ uint32_t load_uniform(const uint32_t *ptr) { return *(const_u32_ptr)(ptr); }
+
+template <bool VEC>
+__device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t L, uint32_t (&w)[4]) {
+    if (VEC) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(src + p);
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t d = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint64_t q = p + j * 4 + b;
+                if (q < L) d |= static_cast<uint32_t>(src[q]) << (8 * b);
+            }
+            w[j] = d;
+        }
+    }
+}
+
+template <bool VEC>
+__device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
+    if (VEC) {
+        *reinterpret_cast<uint4 *>(dst + p) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint64_t q = p + j * 4 + b;
+                if (q < L) dst[q] = static_cast<uint8_t>(w[j] >> (8 * b));
+            }
+    }
+}
+
+// Entry of EB = 2 * RW bytes (RW rows, 2 bytes each).
+template <int EB>
+__device__ __forceinline__ void load_entry(const uint8_t *lds, uint32_t off, uint32_t (&d)[EB / 4]) {
+    if constexpr (EB == 32) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(lds + off);
+        const uint4 b = *reinterpret_cast<const uint4 *>(lds + off + 16);
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    } else if constexpr (EB == 16) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(lds + off);
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+    } else {
+        const uint2 a = *reinterpret_cast<const uint2 *>(lds + off);
+        d[0] = a.x; d[1] = a.y;
+    }
+}
+
+// Fold the 8 elements of a 16-byte slice.  `tab` = the term's 64 entries
+// (nibble position q major, value n minor) of EB bytes.
+template <int EB>
+__device__ __forceinline__ void fold_term16(uint32_t (&acc)[8][EB / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
+    constexpr int SH = EB == 32 ? 5 : (EB == 16 ? 4 : 3);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t x = (w[j] >> (16 * h)) & 0xffffu;
+            uint32_t e0[EB / 4], e1[EB / 4], e2[EB / 4], e3[EB / 4];
+            load_entry<EB>(tab, ((x & 15u) << SH), e0);
+            load_entry<EB>(tab, (16u << SH) + (((x >> 4) & 15u) << SH), e1);
+            load_entry<EB>(tab, (32u << SH) + (((x >> 8) & 15u) << SH), e2);
+            load_entry<EB>(tab, (48u << SH) + ((x >> 12) << SH), e3);
+            uint32_t(&a)[EB / 4] = acc[2 * j + h];
+#pragma unroll
+            for (int d = 0; d < EB / 4; d++) a[d] = xor3(xor3(a[d], e0[d], e1[d]), e2[d], e3[d]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <bool VEC, bool DEC, int RW>
+__device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here, uint64_t p,
+                       bool do_copy) {
+    constexpr int EB = 2 * RW;
+    constexpr int D = EB / 4;
+    uint32_t acc[8][D];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+#pragma unroll
+        for (int d = 0; d < D; d++) acc[e][d] = 0;
+    const bool active = p < a.L;
+    const uint8_t *tab_cw = reinterpret_cast<const uint8_t *>(a.tables + c * a.table_cw_stride);
+    const uint8_t *src_cw = DEC ? nullptr : a.src_base + c * a.src_cw_stride;
+    for (uint32_t t0 = 0; t0 < a.T; t0 += kTermChunk) {
+        const uint32_t tc = min(static_cast<uint32_t>(kTermChunk), a.T - t0);
+        __syncthreads();
+        // Stage: entry (q, n) of term t keeps its first EB bytes (rows 0..RW-1).
+        for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += kThreads) {
+            const uint32_t t = i / (64 * D), rem = i % (64 * D);
+            const uint32_t e = rem / D, d = rem % D;
+            const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
+            reinterpret_cast<uint32_t *>(lds)[i] = *reinterpret_cast<const uint32_t *>(
+                tab_cw + (static_cast<uint64_t>(tt) * a.groups + rg) * kGroupBytes + e * 32 + d * 4);
+        }
+        __syncthreads();
+        if (!active) continue;
+#pragma unroll 1
+        for (uint32_t t = 0; t < tc; t++) {
+            const uint64_t gt = t0 + t;
+            const uint8_t *src = DEC ? load_uniform(a.src_ptrs + c * a.T + gt) : src_cw + gt * a.L;
+            uint32_t w[4];
+            load16<VEC>(src, p, a.L, w);
+            if (DEC && do_copy) {
+                uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + gt);
+                if (cp) store16<VEC>(cp, p, a.L, w);
+            }
+            fold_term16<EB>(acc, lds + t * 64 * EB, w);
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int r = 0; r < RW; r++) {
+        if (r >= rows_here) continue;
+        const int row = rg * kRowsPerPass + r;
+        uint8_t *dst = DEC ? load_uniform(a.dst_ptrs + c * a.rmax + row)
+                           : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(row) * a.L;
+        const int d = r >> 1, sh = (r & 1) * 16;
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            o[j] = ((acc[2 * j][d] >> sh) & 0xffffu) | (((acc[2 * j + 1][d] >> sh) & 0xffffu) << 16);
+        if (a.accumulate) {
+            uint32_t prev[4];
+            load16<VEC>(dst, p, a.L, prev);
+#pragma unroll
+            for (int j = 0; j < 4; j++) o[j] ^= prev[j];
+        }
+        store16<VEC>(dst, p, a.L, o);
+    }
+}
+
+template <bool VEC>
+__device__ void copy_only16(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
+    if (p >= a.L) return;
+    for (uint32_t t = 0; t < a.T; t++) {
+        uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + t);
+        if (!cp) continue;
+        uint32_t w[4];
+        load16<VEC>(load_uniform(a.src_ptrs + c * a.T + t), p, a.L, w);
+        store16<VEC>(cp, p, a.L, w);
+    }
+}
+
+template <bool VEC, bool DEC>
+__global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a, uint32_t tiles, uint32_t units) {
+    __shared__ __align__(16) uint8_t lds[kTermChunk * 64 * 32];
+    // XCD-aware: all tiles of one (codeword, row group) on one XCD.
+    const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const uint32_t unit = xcd + kXcds * (slot / tiles);
+    if (unit >= units) return;
+    const uint32_t tile = slot % tiles;
+    const uint64_t c = unit / a.groups_launched;
+    const int rg = static_cast<int>(unit % a.groups_launched);
+    const uint32_t R = DEC ? load_uniform(a.rows + c) : a.rmax;
+    const uint32_t base = static_cast<uint32_t>(rg) * kRowsPerPass;
+    const int rows_here = static_cast<int>(R > base ? min(static_cast<uint32_t>(kRowsPerPass), R - base) : 0u);
+    const bool do_copy = DEC && rg == 0;
+    if (rows_here == 0 && !do_copy) return;
+    const uint64_t p = static_cast<uint64_t>(tile) * kTile + threadIdx.x * 16;
+    if (DEC && rows_here == 0)
+        copy_only16<VEC>(a, c, p);
+    else if (rows_here <= 4)
+        pass16<VEC, DEC, 4>(a, lds, c, rg, rows_here, p, do_copy);
+    else if (rows_here <= 8)
+        pass16<VEC, DEC, 8>(a, lds, c, rg, rows_here, p, do_copy);
+    else
+        pass16<VEC, DEC, 16>(a, lds, c, rg, rows_here, p, do_copy);
+}
+
+}  // namespace
+
+hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
+    if (args.count == 0 || args.L == 0) return hipSuccess;
+    RowsTermsArgs a = args;
+    const uint64_t tiles = (a.L + kTile - 1) / kTile;
+    const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
+    if (groups > a.groups + (a.rmax == 0)) return hipErrorInvalidValue;
+    a.groups_launched = groups;
+    const uint64_t units = a.count * groups;
+    const uint64_t blocks = (units + kXcds - 1) / kXcds * kXcds * tiles;
+    if (blocks > 0x7fffffffull || units > 0xffffffffull) return hipErrorInvalidValue;
+    const bool dec = a.src_ptrs != nullptr;
+    if (dec != (a.dst_ptrs != nullptr) || dec != (a.copy_ptrs != nullptr) || dec != (a.rows != nullptr))
+        return hipErrorInvalidValue;
+    if (dec && a.accumulate) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<uint32_t>(blocks));
+    const uint32_t t32 = static_cast<uint32_t>(tiles), u32 = static_cast<uint32_t>(units);
+    if (vec16 && dec)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else if (vec16)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else if (dec)
+        hipLaunchKernelGGL((rows_terms16_kernel<false, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else
+        hipLaunchKernelGGL((rows_terms16_kernel<false, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    return hipGetLastError();
+}
+
+}  // namespace scrs

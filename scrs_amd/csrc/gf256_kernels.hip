@@ -155,8 +155,8 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
         for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += kThreads) {
             const uint32_t t = i / (32 * D), rem = i % (32 * D);
             const uint32_t n = rem / D, d = rem % D;
-            reinterpret_cast<uint32_t *>(lds)[i] =
-                tab_cw[((t0 + t) * a.groups + rg) * kGroupDwords + n * 4 + d];
+            const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
+            reinterpret_cast<uint32_t *>(lds)[i] = tab_cw[(tt * a.groups + rg) * kGroupDwords + n * 4 + d];
         }
         __syncthreads();
         if (!active) continue;
@@ -243,6 +243,12 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
             o[j] = ((acc[4 * j + 0][d] >> s) & 0xffu) | (((acc[4 * j + 1][d] >> s) & 0xffu) << 8) |
                    (((acc[4 * j + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * j + 3][d] >> s) & 0xffu) << 24);
         }
+        if (!DEC && a.accumulate) {
+            uint32_t prev[4];
+            load16<VEC>(dst, p, a.L, prev);
+#pragma unroll
+            for (int j = 0; j < 4; j++) o[j] ^= prev[j];
+        }
         store16<VEC>(dst, p, a.L, o);
     }
 }
@@ -291,148 +297,6 @@ __global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Decode solve: one wave64 per codeword.
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ uint8_t gmul(const uint8_t *exp, const uint8_t *log, uint8_t x, uint8_t y) {
-    return (x && y) ? exp[log[x] + log[y]] : 0;
-}
-
-__device__ __forceinline__ uint8_t ginv(const uint8_t *exp, const uint8_t *log, uint8_t x) {
-    return x ? exp[255 - log[x]] : 0;
-}
-
-// GF(2^8) exp (512 entries, doubled) and log tables.
-__constant__ uint8_t c_exp[512];
-__constant__ uint8_t c_log[256];
-int g_tables_device = -1;
-
-__global__ __launch_bounds__(64) void decode_solve_gf256_kernel(DecodeSolveArgs a) {
-    extern __shared__ uint8_t smem[];
-    uint8_t *exp = smem;            // 512
-    uint8_t *log = exp + 512;       // 256
-    uint8_t *seen = log + 256;      // 256
-    uint8_t *rc = seen + 256;       // 256 repair columns, received order
-    uint8_t *ms = rc + 256;         // 256 missing data rows, ascending
-    uint8_t *tp = ms + 256;         // 256 position of received symbol t among the repairs
-    uint8_t *fused = tp + 256;      // rmax x k fused coefficients
-    uint8_t *aug = fused + a.rmax * a.k;  // rmax x (2 * rmax) augmented matrix
-    __shared__ uint32_t s_e;
-    __shared__ uint32_t s_piv;
-
-    const uint64_t c = blockIdx.x;
-    const uint32_t k = a.k, m = a.m;
-    const int lane = threadIdx.x;
-    for (int i = lane; i < 512; i += 64) exp[i] = c_exp[i];
-    for (int i = lane; i < 256; i += 64) { log[i] = c_log[i]; seen[i] = 0; }
-    __syncthreads();
-    const uint16_t *idx = a.indices + c * k;
-    if (lane == 0) {
-        uint32_t e = 0;
-        for (uint32_t t = 0; t < k; t++) {
-            const uint32_t i = idx[t];
-            seen[i] = 1;
-            tp[t] = static_cast<uint8_t>(e);
-            if (i >= k) rc[e++] = static_cast<uint8_t>(i - k);
-        }
-        uint32_t nm = 0;
-        for (uint32_t i = 0; i < k; i++)
-            if (!seen[i]) ms[nm++] = static_cast<uint8_t>(i);
-        s_e = e;
-    }
-    __syncthreads();
-    const uint32_t e = s_e;
-    const uint32_t w2 = 2 * e;
-    // aug[t][s] = C[ms[s]][rc[t]] (the reference's B, batch/codec.rs:383-389),
-    // aug[t][e + s] = identity.
-    for (uint32_t x = lane; x < e * w2; x += 64) {
-        const uint32_t t = x / w2, s = x % w2;
-        aug[t * w2 + s] = s < e ? a.coeffs[ms[s] * m + rc[t]] : static_cast<uint8_t>(s - e == t);
-    }
-    __syncthreads();
-    // Gauss-Jordan; any elimination order yields the unique inverse.
-    for (uint32_t col = 0; col < e; col++) {
-        if (lane == 0) {
-            uint32_t piv = col;
-            while (piv < e && aug[piv * w2 + col] == 0) piv++;
-            s_piv = piv;
-        }
-        __syncthreads();
-        const uint32_t piv = s_piv;
-        if (piv == e) {
-            if (lane == 0) atomicOr(a.error_flag, 1u);
-            return;
-        }
-        if (piv != col)
-            for (uint32_t x = lane; x < w2; x += 64) {
-                const uint8_t tmp = aug[col * w2 + x];
-                aug[col * w2 + x] = aug[piv * w2 + x];
-                aug[piv * w2 + x] = tmp;
-            }
-        __syncthreads();
-        const uint8_t sc = ginv(exp, log, aug[col * w2 + col]);
-        __syncthreads();
-        for (uint32_t x = lane; x < w2; x += 64) aug[col * w2 + x] = gmul(exp, log, aug[col * w2 + x], sc);
-        __syncthreads();
-        for (uint32_t x = lane; x < e * w2; x += 64) {
-            const uint32_t r = x / w2, cc = x % w2;
-            if (r == col || cc == col) continue;
-            const uint8_t f = aug[r * w2 + col];
-            aug[r * w2 + cc] ^= gmul(exp, log, f, aug[col * w2 + cc]);
-        }
-        __syncthreads();
-        for (uint32_t r = lane; r < e; r += 64)
-            if (r != col) aug[r * w2 + col] = 0;
-        __syncthreads();
-    }
-    // Binv[s][u] = aug[s][e + u].  Fused coefficient of received symbol t for
-    // missing row s: Binv[s][pos] for a repair, sum_u Binv[s][u] * C[i][rc[u]]
-    // for a present data row i (the subtract and apply passes of
-    // batch/codec.rs:376-404 folded into one linear map).
-    for (uint32_t x = lane; x < e * k; x += 64) {
-        const uint32_t s = x / k, t = x % k;
-        const uint32_t i = idx[t];
-        uint8_t f;
-        if (i >= k) {
-            f = aug[s * w2 + e + tp[t]];
-        } else {
-            f = 0;
-            for (uint32_t u = 0; u < e; u++) f ^= gmul(exp, log, aug[s * w2 + e + u], a.coeffs[i * m + rc[u]]);
-        }
-        fused[s * k + t] = f;
-    }
-    __syncthreads();
-    // Product tables: [t][g][n][q] dwords, byte s4 of dword q = row 16g+4q+s4.
-    uint32_t *tab = a.tables + c * static_cast<uint64_t>(k) * a.groups * kGroupDwords;
-    for (uint32_t x = lane; x < k * a.groups * kGroupDwords; x += 64) {
-        const uint32_t t = x / (a.groups * kGroupDwords), rem = x % (a.groups * kGroupDwords);
-        const uint32_t g = rem / kGroupDwords, nq = rem % kGroupDwords;
-        const uint32_t n = nq >> 2, q = nq & 3;
-        const uint8_t nib = static_cast<uint8_t>(n < 16 ? n : (n - 16) << 4);
-        uint32_t word = 0;
-        for (uint32_t s4 = 0; s4 < 4; s4++) {
-            const uint32_t s = g * 16 + q * 4 + s4;
-            if (s >= e) break;
-            word |= static_cast<uint32_t>(gmul(exp, log, fused[s * k + t], nib)) << (8 * s4);
-        }
-        tab[x] = word;
-    }
-    const uint64_t L = a.L;
-    uint8_t *out = a.out_base + c * k * L;
-    for (uint32_t t = lane; t < k; t += 64) {
-        const uint32_t i = idx[t];
-        const uint8_t *src;
-        if (a.payload_ptrs) src = a.payload_ptrs[c * k + t];
-        else if (i < k) src = a.data_base + (c * k + i) * L;
-        else src = a.repair_base + (c * m + (i - k)) * L;
-        a.src_ptrs[c * k + t] = src;
-        a.copy_ptrs[c * k + t] = i < k ? out + static_cast<uint64_t>(i) * L : nullptr;
-    }
-    for (uint32_t s = lane; s < e; s += 64) a.dst_ptrs[c * a.rmax + s] = out + static_cast<uint64_t>(ms[s]) * L;
-    if (lane == 0) a.rows[c] = e;
-}
-
 __global__ void fill_splitmix_kernel(uint8_t *dst, uint64_t len, uint64_t seed, uint64_t word0) {
     const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -479,37 +343,6 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStrea
         hipLaunchKernelGGL((rows_terms_lds_kernel<false, true, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else
         hipLaunchKernelGGL((rows_terms_lds_kernel<false, false, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
-    return hipGetLastError();
-}
-
-static hipError_t ensure_field_tables() {
-    int dev = -1;
-    hipError_t err = hipGetDevice(&dev);
-    if (err != hipSuccess) return err;
-    if (g_tables_device == dev) return hipSuccess;
-    uint8_t exp[512], log[256] = {0};
-    uint8_t v = 1;
-    for (int i = 0; i < 255; i++) {
-        exp[i] = exp[i + 255] = v;
-        log[v] = static_cast<uint8_t>(i);
-        v = static_cast<uint8_t>(v ^ ((v << 1) ^ ((v & 0x80) ? 0x1B : 0)));
-    }
-    exp[510] = exp[0];
-    exp[511] = exp[1];
-    err = hipMemcpyToSymbol(HIP_SYMBOL(c_exp), exp, sizeof(exp));
-    if (err != hipSuccess) return err;
-    err = hipMemcpyToSymbol(HIP_SYMBOL(c_log), log, sizeof(log));
-    if (err == hipSuccess) g_tables_device = dev;
-    return err;
-}
-
-hipError_t launch_decode_solve_gf256(const DecodeSolveArgs &a, hipStream_t s) {
-    if (a.count == 0) return hipSuccess;
-    hipError_t err = ensure_field_tables();
-    if (err != hipSuccess) return err;
-    const size_t shmem = 512 + 256 * 5 + static_cast<size_t>(a.rmax) * a.k + static_cast<size_t>(a.rmax) * 2 * a.rmax;
-    if (shmem > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(decode_solve_gf256_kernel, dim3(static_cast<uint32_t>(a.count)), dim3(64), shmem, s, a);
     return hipGetLastError();
 }
 

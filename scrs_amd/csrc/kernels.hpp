@@ -1,4 +1,4 @@
-// kernels.hpp - launch interfaces of the GF(2^8) device kernels.
+// kernels.hpp - launch interfaces of the device kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,7 +15,8 @@ namespace scrs {
 // (dst_base + c * dst_cw_stride + r * L, or dst_ptrs[c * rmax + r]).  When
 // copy_ptrs is set, copy_ptrs[c * T + t] (if non-null) receives a verbatim
 // copy of source t: decode writes its present data rows this way while it
-// streams them as terms.  Destinations are overwritten, never accumulated.
+// streams them as terms.  Destinations are overwritten unless `accumulate`
+// is set, in which case the sum is XORed into them (incremental encode).
 struct RowsTermsArgs {
     const uint8_t *src_base = nullptr;
     uint64_t src_cw_stride = 0;
@@ -31,6 +32,8 @@ struct RowsTermsArgs {
     uint32_t rmax = 0;                // rows (upper bound when `rows` is set)
     uint32_t groups = 0;              // 16-row groups per term in the tables
     uint32_t groups_launched = 0;     // row groups this launch covers (set by the launcher)
+    uint32_t accumulate = 0;          // XOR into the destinations instead of overwriting
+    const uint32_t *term_index = nullptr; // table term of source t (device, [T]), or null: t
     uint64_t L = 0;                   // symbol length in bytes
     uint64_t count = 0;               // codewords
 };
@@ -39,29 +42,52 @@ struct RowsTermsArgs {
 // `vec16` requires every row address and L to be multiples of 16.
 hipError_t launch_rows_terms_lds(const RowsTermsArgs &a, bool vec16, hipStream_t s);
 
-// Per-codeword decode solve.  For each codeword, from the k received
-// indices: the missing data rows, the e x e inverse of the Cauchy
-// submatrix, and the fused e x k coefficients applied directly to the k
-// received symbols.  Writes packed nibble tables, row counts, and the
-// source / destination / copy pointer tables consumed by the multiply-XOR.
-struct DecodeSolveArgs {
-    const uint16_t *indices = nullptr;   // [count][k], received order
-    const uint8_t *coeffs = nullptr;     // k x m source-major
-    const uint8_t *const *payload_ptrs = nullptr; // [count][k] or null: layout mode
-    const uint8_t *data_base = nullptr;  // layout mode: data rows c*k*L + i*L
-    const uint8_t *repair_base = nullptr;// layout mode: repair rows c*m*L + j*L
-    uint8_t *out_base = nullptr;         // out rows c*k*L + i*L
-    uint32_t *tables = nullptr;          // [count][k][groups][32][4]
-    uint32_t *rows = nullptr;            // [count]
-    const uint8_t **src_ptrs = nullptr;  // [count][k]
-    uint8_t **dst_ptrs = nullptr;        // [count][rmax]
-    uint8_t **copy_ptrs = nullptr;       // [count][k]
-    uint32_t *error_flag = nullptr;      // set non-zero if a submatrix is singular
+// Closed-form Cauchy solve for a batch of codewords (either field).  For
+// codeword c with received indices[c][0..k): rows[c] = r (missing data rows),
+// missing[c][0..r) ascending, fused[c][j][t] = coefficient of received symbol
+// t for missing row j.  coords[i] is the field coordinate of codeword index
+// i (x for data, y for repairs).  `work` holds cauchy_solve_work_elems(k,
+// rmax) u16 per codeword.
+struct CauchySolveArgs {
+    const uint16_t *indices = nullptr;
+    const uint16_t *coords = nullptr;
+    uint16_t *fused = nullptr;     // [count][rmax][k]
+    uint16_t *missing = nullptr;   // [count][rmax]
+    uint32_t *rows = nullptr;      // [count]
+    uint16_t *work = nullptr;      // [count][work_stride]
+    uint64_t work_stride = 0;
+    uint32_t *error_flag = nullptr;
+    uint32_t k = 0, m = 0, rmax = 0;
+    uint64_t count = 0;
+};
+uint64_t cauchy_solve_work_elems(uint32_t k, uint32_t rmax);
+hipError_t launch_cauchy_solve(const CauchySolveArgs &a, bool gf16, hipStream_t s);
+
+// Product tables and pointer tables from a solve.  GF(2^8) tables use the
+// RowsTermsArgs layout (512 B per term and 16-row group); GF(2^16) tables
+// use 2048 B per term and group: [q nibble][n][16 rows x 2 bytes].
+struct DecodePrepArgs {
+    const uint16_t *indices = nullptr;
+    const uint16_t *fused = nullptr;
+    const uint16_t *missing = nullptr;
+    const uint32_t *rows = nullptr;
+    const uint8_t *const *payload_ptrs = nullptr;
+    const uint8_t *data_base = nullptr;
+    const uint8_t *repair_base = nullptr;
+    uint8_t *out_base = nullptr;
+    uint32_t *tables = nullptr;
+    const uint8_t **src_ptrs = nullptr;
+    uint8_t **dst_ptrs = nullptr;
+    uint8_t **copy_ptrs = nullptr;
     uint32_t k = 0, m = 0, rmax = 0, groups = 0;
     uint64_t L = 0, count = 0;
 };
+hipError_t launch_decode_prepare(const DecodePrepArgs &a, bool gf16, hipStream_t s);
 
-hipError_t launch_decode_solve_gf256(const DecodeSolveArgs &a, hipStream_t s);
+// GF(2^16) multiply-XOR over interleaved little-endian u16 elements, same
+// addressing as RowsTermsArgs; tables in the 2048 B per (term, group) layout
+// (build_nibble_tables16).
+hipError_t launch_rows_terms16(const RowsTermsArgs &a, bool vec16, hipStream_t s);
 
 // Fill `len` bytes with the seeded splitmix64 counter stream: byte i is byte
 // (i % 8) of splitmix64 word i / 8 drawn from state seed + (i/8) * golden.

@@ -77,4 +77,27 @@ void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t groups,
         }
 }
 
+void build_nibble_tables16(const uint16_t *coef, size_t T, size_t R, size_t groups,
+                           uint32_t *out) {
+    for (size_t t = 0; t < T; t++)
+        for (size_t g = 0; g < groups; g++) {
+            uint8_t *tab = reinterpret_cast<uint8_t *>(out + (t * groups + g) * 512);
+            for (int s = 0; s < 16; s++) {
+                const size_t r = g * 16 + s;
+                const uint16_t c = r < R ? coef[t * R + r] : 0;
+                uint16_t basis[16];
+                for (int b = 0; b < 16; b++) basis[b] = Gf16::mul(c, static_cast<uint16_t>(1u << b));
+                for (int q = 0; q < 4; q++)
+                    for (int n = 0; n < 16; n++) {
+                        uint16_t v = 0;
+                        for (int b = 0; b < 4; b++)
+                            if ((n >> b) & 1) v ^= basis[4 * q + b];
+                        uint8_t *e = tab + (q * 16 + n) * 32 + 2 * s;
+                        e[0] = static_cast<uint8_t>(v);
+                        e[1] = static_cast<uint8_t>(v >> 8);
+                    }
+            }
+        }
+}
+
 }  // namespace scrs

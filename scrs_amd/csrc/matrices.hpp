@@ -35,4 +35,11 @@ std::vector<uint16_t> tower_cauchy_matrix(size_t k, size_t m);
 void build_nibble_tables(const uint8_t *coef, size_t T, size_t R, size_t groups,
                          uint32_t *out);
 
+// GF(2^16) product tables for a T x R block (coef[t*R + r]).  For term t and
+// 16-row group g, the 2048 bytes at dword (t * groups + g) * 512 hold 64
+// entries of 32 bytes, entry (q * 16 + n) being the row products
+// coef[t][16g + s] * (n << 4q), little-endian 2 bytes per row s.
+void build_nibble_tables16(const uint16_t *coef, size_t T, size_t R, size_t groups,
+                           uint32_t *out);
+
 }  // namespace scrs

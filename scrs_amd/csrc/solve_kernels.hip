@@ -1,0 +1,324 @@
+// solve_kernels.hip - per-codeword Cauchy solve and decode preparation.
+//
+// For the k received symbols of a codeword, the r missing data rows are
+// recovered from the r received repairs through the inverse of the r x r
+// Cauchy submatrix A[i][j] = 1 / (row_i + col_j) (row = repair coordinate,
+// col = missing data coordinate), fused with the contributions of the k - r
+// present data rows.  The closed form is the reference's rational-Lagrange
+// factorisation (scrs/src/decoder/cauchy_inverse.rs:29-170,
+// scrs/src/tower/decoder.rs:433-546):
+//   R_i = prod_h (row_i + col_h) / prod_{l != i} (row_i + row_l)
+//   C_j = prod_l (col_j + row_l) / prod_{h != j} (col_j + col_h)
+//   A^-1[j][i] = R_i C_j / (col_j + row_i)
+//   present z:  F[j][z] = C_j * Q(z) / (z + col_j),
+//               Q(z) = prod_h (z + col_h) / prod_l (z + row_l)
+// Every entry is independent, so the whole solve is data parallel.  The
+// fused coefficients are then expanded into split-nibble product tables.
+#include <hip/hip_runtime.h>
+
+#include "gf_device.hpp"
+#include "kernels.hpp"
+
+namespace scrs {
+namespace {
+
+constexpr int kSolveThreads = 256;
+
+__constant__ uint8_t c_exp8[512];
+__constant__ uint8_t c_log8[256];
+int g_solve_tables_device = -1;
+
+__device__ __forceinline__ void stage_gf8(uint8_t *exp, uint8_t *log) {
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) exp[i] = c_exp8[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) log[i] = c_log8[i];
+}
+
+// Block-wide exclusive prefix sum of one 0/1 flag per thread; returns this
+// thread's offset and writes the block total to *total.
+__device__ __forceinline__ uint32_t block_scan(uint32_t flag, uint32_t *buf, uint32_t *total) {
+    buf[threadIdx.x] = flag;
+    __syncthreads();
+    for (uint32_t off = 1; off < kSolveThreads; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? buf[threadIdx.x - off] : 0u;
+        __syncthreads();
+        buf[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t incl = buf[threadIdx.x];
+    *total = buf[kSolveThreads - 1];
+    __syncthreads();
+    return incl - flag;
+}
+
+template <typename F>
+__global__ __launch_bounds__(kSolveThreads) void cauchy_solve_kernel(CauchySolveArgs a) {
+    __shared__ uint8_t exp[512];
+    __shared__ uint8_t log[256];
+    __shared__ uint32_t scan[kSolveThreads];
+    __shared__ uint32_t s_r, s_e;
+    extern __shared__ uint32_t seen[];  // n bits
+    const F f{{exp, log}};
+    const uint64_t c = blockIdx.x;
+    const uint32_t k = a.k, n = a.k + a.m, r_cap = a.rmax;
+    const uint16_t *idx = a.indices + c * k;
+    stage_gf8(exp, log);
+    for (uint32_t w = threadIdx.x; w < (n + 31) / 32; w += kSolveThreads) seen[w] = 0;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < k; t += kSolveThreads) {
+        const uint32_t i = idx[t];
+        atomicOr(&seen[i >> 5], 1u << (i & 31));
+    }
+    if (threadIdx.x == 0) { s_r = 0; s_e = 0; }
+    __syncthreads();
+    uint16_t *rpos = a.work + c * a.work_stride;  // [k] received order -> repair rank / present rank
+    uint32_t *rv = reinterpret_cast<uint32_t *>(rpos + ((k + 1) & ~1u));  // [rmax] row (repair) coordinates
+    uint32_t *cv = rv + r_cap;          // [rmax] column (missing data) coordinates
+    uint32_t *rf = cv + r_cap;          // [rmax] row factors R_i
+    uint32_t *cf = rf + r_cap;          // [rmax] column factors C_j
+    uint32_t *q = cf + r_cap;           // [k] Q(z) for present data, by received position
+    uint16_t *missing = a.missing + c * r_cap;
+    // Received repairs, in received order, become the rows.
+    for (uint32_t t0 = 0; t0 < k; t0 += kSolveThreads) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t flag = (t < k && idx[t] >= k) ? 1u : 0u;
+        uint32_t total;
+        const uint32_t off = block_scan(flag, scan, &total);
+        if (flag) {
+            const uint32_t i = s_r + off;
+            if (i < r_cap) {
+                rpos[t] = static_cast<uint16_t>(i);
+                rv[i] = a.coords[idx[t]];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_r += total;
+        __syncthreads();
+    }
+    // Missing data rows, ascending, become the columns.
+    for (uint32_t i0 = 0; i0 < k; i0 += kSolveThreads) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t flag = (i < k && !((seen[i >> 5] >> (i & 31)) & 1u)) ? 1u : 0u;
+        uint32_t total;
+        const uint32_t off = block_scan(flag, scan, &total);
+        if (flag) {
+            const uint32_t j = s_e + off;
+            if (j < r_cap) {
+                missing[j] = static_cast<uint16_t>(i);
+                cv[j] = a.coords[i];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_e += total;
+        __syncthreads();
+    }
+    const uint32_t r = s_r;
+    if (r != s_e || r > r_cap) {
+        if (threadIdx.x == 0) atomicOr(a.error_flag, 1u);
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < r; i += kSolveThreads) {
+        uint32_t cross = 1, within = 1;
+        const uint32_t x = rv[i];
+        for (uint32_t h = 0; h < r; h++) cross = f.mul(cross, x ^ cv[h]);
+        for (uint32_t l = 0; l < r; l++)
+            if (l != i) within = f.mul(within, x ^ rv[l]);
+        rf[i] = f.mul(cross, f.inv(within));
+    }
+    for (uint32_t j = threadIdx.x; j < r; j += kSolveThreads) {
+        uint32_t cross = 1, within = 1;
+        const uint32_t x = cv[j];
+        for (uint32_t l = 0; l < r; l++) cross = f.mul(cross, x ^ rv[l]);
+        for (uint32_t h = 0; h < r; h++)
+            if (h != j) within = f.mul(within, x ^ cv[h]);
+        cf[j] = f.mul(cross, f.inv(within));
+    }
+    for (uint32_t t = threadIdx.x; t < k; t += kSolveThreads) {
+        const uint32_t i = idx[t];
+        if (i >= k) continue;
+        const uint32_t z = a.coords[i];
+        uint32_t num = 1, den = 1;
+        for (uint32_t h = 0; h < r; h++) num = f.mul(num, z ^ cv[h]);
+        for (uint32_t l = 0; l < r; l++) den = f.mul(den, z ^ rv[l]);
+        q[t] = f.mul(num, f.inv(den));
+    }
+    __syncthreads();
+    uint16_t *fused = a.fused + c * static_cast<uint64_t>(r_cap) * k;
+    for (uint32_t x = threadIdx.x; x < r * k; x += kSolveThreads) {
+        const uint32_t j = x / k, t = x % k;
+        const uint32_t i = idx[t];
+        uint32_t v;
+        if (i >= k) {
+            const uint32_t row = rpos[t];
+            v = f.mul(f.mul(rf[row], cf[j]), f.inv(cv[j] ^ rv[row]));
+        } else {
+            v = f.mul(f.mul(cf[j], q[t]), f.inv(a.coords[i] ^ cv[j]));
+        }
+        fused[j * k + t] = static_cast<uint16_t>(v);
+    }
+    if (threadIdx.x == 0) a.rows[c] = r;
+}
+
+// Split-nibble product tables for GF(2^8) from the fused coefficients:
+// [c][t][g][n][16 rows] bytes, entry n < 16 = F * n, 16 + n = F * (n << 4).
+// One thread per (t, g, row): eight basis products F * 2^b, then the 32
+// entry bytes are XOR combinations of them.
+__global__ __launch_bounds__(256) void tables_gf256_kernel(DecodePrepArgs a) {
+    __shared__ uint8_t exp[512];
+    __shared__ uint8_t log[256];
+    stage_gf8(exp, log);
+    __syncthreads();
+    const Gf8Tables t8{exp, log};
+    const uint64_t c = blockIdx.y;
+    const uint32_t k = a.k, groups = a.groups;
+    const uint32_t r = a.rows[c];
+    const uint64_t items = static_cast<uint64_t>(k) * groups * 16;
+    uint8_t *tab = reinterpret_cast<uint8_t *>(a.tables + c * static_cast<uint64_t>(k) * groups * 128);
+    const uint16_t *fused = a.fused + c * static_cast<uint64_t>(a.rmax) * k;
+    for (uint64_t x = blockIdx.x * 256ull + threadIdx.x; x < items; x += 256ull * gridDim.x) {
+        const uint32_t s = x % 16, g = (x / 16) % groups, t = x / (16 * groups);
+        const uint32_t row = g * 16 + s;
+        const uint32_t fv = row < r ? fused[row * k + t] : 0u;
+        uint32_t basis[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) basis[b] = t8.mul(fv, 1u << b);
+        uint8_t *e = tab + (static_cast<uint64_t>(t) * groups + g) * 512 + s;
+#pragma unroll
+        for (int nib = 0; nib < 16; nib++) {
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if ((nib >> b) & 1) { lo ^= basis[b]; hi ^= basis[4 + b]; }
+            e[nib * 16] = static_cast<uint8_t>(lo);
+            e[(16 + nib) * 16] = static_cast<uint8_t>(hi);
+        }
+    }
+}
+
+// GF(2^16) tables: [c][t][g][q (4 nibbles)][n (16)][16 rows x 2 bytes].
+__global__ __launch_bounds__(256) void tables_gf65536_kernel(DecodePrepArgs a) {
+    __shared__ uint8_t exp[512];
+    __shared__ uint8_t log[256];
+    stage_gf8(exp, log);
+    __syncthreads();
+    const FieldGf65536 f{{exp, log}};
+    const uint64_t c = blockIdx.y;
+    const uint32_t k = a.k, groups = a.groups;
+    const uint32_t r = a.rows[c];
+    const uint64_t items = static_cast<uint64_t>(k) * groups * 16;
+    uint8_t *tab = reinterpret_cast<uint8_t *>(a.tables + c * static_cast<uint64_t>(k) * groups * 512);
+    const uint16_t *fused = a.fused + c * static_cast<uint64_t>(a.rmax) * k;
+    for (uint64_t x = blockIdx.x * 256ull + threadIdx.x; x < items; x += 256ull * gridDim.x) {
+        const uint32_t s = x % 16, g = (x / 16) % groups, t = x / (16 * groups);
+        const uint32_t row = g * 16 + s;
+        const uint32_t fv = row < r ? fused[row * k + t] : 0u;
+        uint32_t basis[16];
+#pragma unroll
+        for (int b = 0; b < 16; b++) basis[b] = f.mul(fv, 1u << b);
+        uint8_t *e = tab + (static_cast<uint64_t>(t) * groups + g) * 2048 + 2 * s;
+#pragma unroll
+        for (int qd = 0; qd < 4; qd++)
+#pragma unroll
+            for (int nib = 0; nib < 16; nib++) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if ((nib >> b) & 1) v ^= basis[4 * qd + b];
+                uint8_t *p = e + (qd * 16 + nib) * 32;
+                p[0] = static_cast<uint8_t>(v);
+                p[1] = static_cast<uint8_t>(v >> 8);
+            }
+    }
+}
+
+// Pointer tables of the multiply-XOR: received payloads, copies of present
+// data rows, destinations of the missing rows.
+__global__ __launch_bounds__(256) void pointers_kernel(DecodePrepArgs a) {
+    const uint64_t c = blockIdx.x;
+    const uint32_t k = a.k, m = a.m;
+    const uint64_t L = a.L;
+    const uint16_t *idx = a.indices + c * k;
+    uint8_t *out = a.out_base + c * k * L;
+    for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
+        const uint32_t i = idx[t];
+        const uint8_t *src;
+        if (a.payload_ptrs) src = a.payload_ptrs[c * k + t];
+        else if (i < k) src = a.data_base + (c * k + i) * L;
+        else src = a.repair_base + (c * m + (i - k)) * L;
+        a.src_ptrs[c * k + t] = src;
+        a.copy_ptrs[c * k + t] = i < k ? out + static_cast<uint64_t>(i) * L : nullptr;
+    }
+    const uint32_t r = a.rows[c];
+    const uint16_t *missing = a.missing + c * a.rmax;
+    for (uint32_t s = threadIdx.x; s < r; s += blockDim.x)
+        a.dst_ptrs[c * a.rmax + s] = out + static_cast<uint64_t>(missing[s]) * L;
+}
+
+}  // namespace
+
+void gf8_exp_log(uint8_t exp[512], uint8_t log[256]) {
+    uint8_t v = 1;
+    for (int i = 0; i < 256; i++) log[i] = 0;
+    for (int i = 0; i < 255; i++) {
+        exp[i] = exp[i + 255] = v;
+        log[v] = static_cast<uint8_t>(i);
+        v = static_cast<uint8_t>(v ^ ((v << 1) ^ ((v & 0x80) ? 0x1B : 0)));
+    }
+    exp[510] = exp[0];
+    exp[511] = exp[1];
+}
+
+static hipError_t ensure_solve_tables() {
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (g_solve_tables_device == dev) return hipSuccess;
+    uint8_t exp[512], log[256];
+    gf8_exp_log(exp, log);
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_exp8), exp, sizeof(exp))) != hipSuccess) return err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_log8), log, sizeof(log))) != hipSuccess) return err;
+    g_solve_tables_device = dev;
+    return hipSuccess;
+}
+
+uint64_t cauchy_solve_work_elems(uint32_t k, uint32_t rmax) {
+    // rpos (k u16, padded to even) + rv, cv, rf, cf (rmax u32 each) + q (k u32), in u16 units.
+    return ((k + 1) & ~1ull) + 2ull * (4ull * rmax + k);
+}
+
+hipError_t launch_cauchy_solve(const CauchySolveArgs &a, bool gf16, hipStream_t s) {
+    if (a.count == 0) return hipSuccess;
+    hipError_t err = ensure_solve_tables();
+    if (err != hipSuccess) return err;
+    const size_t shmem = ((a.k + a.m + 31) / 32) * 4;
+    if (shmem > 48 * 1024 || a.count > 0x7fffffffull) return hipErrorInvalidValue;
+    if (gf16)
+        hipLaunchKernelGGL(cauchy_solve_kernel<FieldGf65536>, dim3(static_cast<uint32_t>(a.count)),
+                           dim3(kSolveThreads), shmem, s, a);
+    else
+        hipLaunchKernelGGL(cauchy_solve_kernel<FieldGf256>, dim3(static_cast<uint32_t>(a.count)),
+                           dim3(kSolveThreads), shmem, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_prepare(const DecodePrepArgs &a, bool gf16, hipStream_t s) {
+    if (a.count == 0) return hipSuccess;
+    hipError_t err = ensure_solve_tables();
+    if (err != hipSuccess) return err;
+    const uint64_t items = static_cast<uint64_t>(a.k) * a.groups * 16;
+    const uint32_t bx = static_cast<uint32_t>((items + 255) / 256 < 64 ? (items + 255) / 256 : 64);
+    if (a.count > 65535) return hipErrorInvalidValue;
+    const dim3 grid(bx, static_cast<uint32_t>(a.count));
+    // Without fused coefficients the tables already exist (a cached
+    // recipe): only the pointer tables are written.
+    if (a.fused) {
+        if (gf16)
+            hipLaunchKernelGGL(tables_gf65536_kernel, grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(tables_gf256_kernel, grid, dim3(256), 0, s, a);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(pointers_kernel, dim3(static_cast<uint32_t>(a.count)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace scrs
