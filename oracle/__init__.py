@@ -78,6 +78,9 @@ def lib() -> C.CDLL:
         L.or_gf256_decode_many.argtypes = [_u8p, C.c_size_t, C.c_size_t, C.c_size_t, _u16p, _u8p, _u8p, _u8p,
                                            C.c_size_t, C.c_int]
         L.or_fill_bytes.argtypes = [C.c_uint64, C.c_uint64, _u8p, C.c_size_t]
+        L.or_rs16_interp.restype = C.c_int
+        L.or_rs16_interp.argtypes = [C.c_size_t, C.c_size_t, _szp, C.POINTER(C.c_void_p), C.c_size_t, _szp, _u8p]
+        L.or_afft_encode.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _u8p, _u8p]
         _lib = L
     return _lib
 
@@ -267,6 +270,36 @@ def tower_decode_rref(coeffs: np.ndarray, k: int, m: int, L: int, symbols) -> np
     out = np.zeros(k * L, np.uint8)
     rank = lib().or_tower_decode_rref(_p16(c), k, m, L, idx, ptrs, _p8(out))
     return out if rank == k else None
+
+
+# ---- additive-FFT codec (Reed-Solomon at points = wire index) ---------------
+
+def rs16_interp(k: int, L: int, points, values, outs) -> np.ndarray:
+    """Evaluate the degree < k polynomial through (points[t], values[t]) at `outs`."""
+    keep = [np.ascontiguousarray(np.frombuffer(bytes(v), np.uint8)) if not isinstance(v, np.ndarray)
+            else np.ascontiguousarray(v, np.uint8).reshape(-1) for v in values]
+    assert len(keep) == k and all(v.size == L for v in keep)
+    pts = (C.c_size_t * max(k, 1))(*[int(p) for p in points])
+    vals = (C.c_void_p * max(k, 1))(*[v.ctypes.data for v in keep])
+    outs = list(outs)
+    o = (C.c_size_t * max(len(outs), 1))(*[int(x) for x in outs])
+    out = np.zeros(max(len(outs) * L, 1), np.uint8)
+    if not lib().or_rs16_interp(k, L, pts, vals, len(outs), o, _p8(out)):
+        raise ValueError("points are not distinct")
+    return out[:len(outs) * L]
+
+
+def afft_encode(k: int, m: int, L: int, data: np.ndarray) -> np.ndarray:
+    d = np.ascontiguousarray(data, np.uint8).reshape(-1)
+    assert d.size == k * L
+    out = np.zeros(m * L, np.uint8)
+    lib().or_afft_encode(k, m, L, _p8(d), _p8(out))
+    return out
+
+
+def afft_decode(k: int, L: int, symbols) -> np.ndarray:
+    """Data symbols from any k received (index, payload) pairs."""
+    return rs16_interp(k, L, [s[0] for s in symbols], [s[1] for s in symbols], range(k))
 
 
 # ---- seeded inputs --------------------------------------------------------

@@ -411,6 +411,105 @@ int or_tower_decode_rref(const uint16_t *coeffs, size_t k, size_t m, size_t L,
 }
 
 /* ====================================================================== */
+/* GF(65536) additive-FFT codec: Reed-Solomon at points = wire index       */
+/* ====================================================================== */
+
+static uint16_t g16_exp[2 * 65535];
+static uint32_t g16_log[65536];
+static pthread_once_t g16_once = PTHREAD_ONCE_INIT;
+
+/* log/exp over the generator 0x0108 (algebra/gf65536.rs:26), whose order
+ * 65535 the KAT suite checks. */
+static void g16_tables(void) {
+    uint16_t v = 1;
+    for (uint32_t i = 0; i < 65535; i++) {
+        g16_exp[i] = g16_exp[i + 65535] = v;
+        g16_log[v] = i;
+        v = or_gf16_mul(v, 0x0108);
+    }
+}
+
+static inline uint16_t g16_mul(uint16_t a, uint16_t b) {
+    return (a && b) ? g16_exp[g16_log[a] + g16_log[b]] : 0;
+}
+
+static inline uint16_t g16_div(uint16_t a, uint16_t b) {
+    return a ? g16_exp[g16_log[a] + 65535 - g16_log[b]] : 0;
+}
+
+/* Lagrange: P(x) = sum_t v_t A(x) / ((x + p_t) W_t), A(x) = prod_s (x + p_s),
+ * W_t = prod_{s != t} (p_t + p_s).  When the points are exactly 0..k-1 with
+ * k a power of two they form an additive subgroup, p_t + p_s runs over its
+ * nonzero elements, and every W_t is their product. */
+int or_rs16_interp(size_t k, size_t L, const size_t *pts, const uint8_t *const *vals,
+                   size_t nout, const size_t *outs, uint8_t *out) {
+    pthread_once(&g16_once, g16_tables);
+    uint16_t *w = (uint16_t *)malloc(k * sizeof(uint16_t));
+    uint16_t *coef = (uint16_t *)malloc(k * sizeof(uint16_t));
+    int subgroup = (k & (k - 1)) == 0;
+    for (size_t t = 0; t < k && subgroup; t++) subgroup = pts[t] == t;
+    if (subgroup) {
+        uint16_t prod = 1;
+        for (size_t u = 1; u < k; u++) prod = g16_mul(prod, (uint16_t)u);
+        for (size_t t = 0; t < k; t++) w[t] = prod;
+    } else {
+        for (size_t t = 0; t < k; t++) {
+            uint16_t prod = 1;
+            for (size_t s = 0; s < k; s++)
+                if (s != t) prod = g16_mul(prod, (uint16_t)(pts[t] ^ pts[s]));
+            if (!prod) {
+                free(w);
+                free(coef);
+                return 0;
+            }
+            w[t] = prod;
+        }
+    }
+    for (size_t i = 0; i < nout; i++) {
+        const size_t x = outs[i];
+        uint8_t *o = out + i * L;
+        size_t hit = k;
+        uint16_t a = 1;
+        for (size_t s = 0; s < k; s++) {
+            if (pts[s] == x) hit = s;
+            else a = g16_mul(a, (uint16_t)(x ^ pts[s]));
+        }
+        if (hit < k) {
+            memcpy(o, vals[hit], L);
+            continue;
+        }
+        for (size_t t = 0; t < k; t++) coef[t] = g16_div(a, g16_mul((uint16_t)(x ^ pts[t]), w[t]));
+        memset(o, 0, L);
+        for (size_t t = 0; t < k; t++) {
+            const uint32_t lc = g16_log[coef[t]];
+            const uint8_t *v = vals[t];
+            for (size_t p = 0; p + 1 < L; p += 2) {
+                const uint16_t e = ld16(v + p);
+                if (e) st16(o + p, (uint16_t)(ld16(o + p) ^ g16_exp[lc + g16_log[e]]));
+            }
+        }
+    }
+    free(w);
+    free(coef);
+    return 1;
+}
+
+void or_afft_encode(size_t k, size_t m, size_t L, const uint8_t *data, uint8_t *repairs) {
+    size_t *pts = (size_t *)malloc(k * sizeof(size_t));
+    size_t *outs = (size_t *)malloc(m * sizeof(size_t));
+    const uint8_t **vals = (const uint8_t **)malloc(k * sizeof(uint8_t *));
+    for (size_t i = 0; i < k; i++) {
+        pts[i] = i;
+        vals[i] = data + i * L;
+    }
+    for (size_t j = 0; j < m; j++) outs[j] = k + j;
+    or_rs16_interp(k, L, pts, vals, m, outs, repairs);
+    free(pts);
+    free(outs);
+    free(vals);
+}
+
+/* ====================================================================== */
 /* CPU baseline: register-blocked source-major multiply-XOR                */
 /* ====================================================================== */
 

@@ -82,6 +82,21 @@ int or_tower_decode_rref(const uint16_t *coeffs, size_t k, size_t m, size_t L,
                          const size_t *idx, const uint8_t *const *payloads,
                          uint8_t *out);
 
+/* ---- GF(65536) additive-FFT codec (afft/) ----
+ * The AFFT engine is Reed-Solomon over GF(2^16) evaluated at the points
+ * x = wire index (afft/profile.rs:41-46): data symbol i is P(i), repair j is
+ * P(k + j), deg P < k.  Whatever transform computes it, the bytes are fixed
+ * by that polynomial; the oracle evaluates it by Lagrange interpolation, the
+ * closed form the reference's own encoder test uses (afft/encoder.rs:347-382). */
+
+/* Evaluate the polynomial of degree < k through (pts[t], vals[t]), t < k,
+ * at the `nout` points outs[i], writing L-byte symbols to out + i*L.
+ * Returns 0 if the points are not distinct, else 1. */
+int or_rs16_interp(size_t k, size_t L, const size_t *pts, const uint8_t *const *vals,
+                   size_t nout, const size_t *outs, uint8_t *out);
+/* Systematic encode: repairs[j] = P(k + j) with P(i) = data_i. */
+void or_afft_encode(size_t k, size_t m, size_t L, const uint8_t *data, uint8_t *repairs);
+
 /* ---- CPU baseline: the reference's x86 multiply-XOR plan ---- */
 /* Returns the backend chosen: 2 = GFNI+AVX2, 1 = AVX2 nibble, 0 = scalar. */
 int or_cpu_plan(void);

@@ -1,0 +1,143 @@
+// afft.hpp - device additive FFT over GF(2^16): kernels and the codec engine.
+//
+// Symbols are rows of L bytes holding L/2 little-endian GF(2^16) elements;
+// every element column is an independent transform over the rows.  A pass
+// loads a group of 2^g rows (the rows that differ only in g consecutive bits
+// of the row index) for a 256-byte column tile into registers and LDS, runs
+// its butterfly levels and writes it back.  A transform over 2^D rows takes
+// ceil(D / 8) passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "../../include/scrs.h"
+#include "device.hpp"
+
+namespace scrs {
+
+enum AfftPassFlags : uint32_t {
+    kAfftInv = 1,    // inverse levels over the pass bits (bottom-up)
+    kAfftDeriv = 2,  // formal derivative contributions of the pass bits
+    kAfftFwd = 4,    // forward levels over the pass bits (top-down)
+};
+
+struct AfftPassArgs {
+    // Input rows: in_rows[r] (null = zero row), else in_base + r*L for
+    // r < in_limit and zero above.  Codeword z adds z*in_cw_stride.
+    const uint8_t *const *in_rows = nullptr;
+    const uint8_t *in_base = nullptr;
+    uint64_t in_limit = 0, in_cw_stride = 0;
+    // Derivative accumulator XORed into the derivative (rows of aux_base).
+    const uint8_t *aux_base = nullptr;
+    uint64_t aux_cw_stride = 0;
+    // Output rows: out_rows[r] (null = not stored), else out_base +
+    // (r - out_first)*L for out_first <= r < out_limit.
+    uint8_t *const *out_rows = nullptr;
+    uint8_t *out_base = nullptr;
+    uint64_t out_first = 0, out_limit = 0, out_cw_stride = 0;
+    // Loaded rows are also copied, unscaled, to copy_rows[r] when non-null.
+    uint8_t *const *copy_rows = nullptr;
+    // Per-row constant tables (64 u16 each, row r at r*64) applied after
+    // the load / before the store.
+    const uint16_t *scale_in = nullptr;
+    const uint16_t *scale_out = nullptr;
+    const uint16_t *node_tabs = nullptr;   // 65536 node tables
+    const uint16_t *deriv_tabs = nullptr;  // 16 derivative tables
+    uint64_t L = 0;
+    uint32_t b_lo = 0, g = 0;              // pass bits [b_lo, b_lo + g)
+    uint32_t inv_base = 0, fwd_base = 0;   // point of buffer row 0, per direction
+    uint32_t flags = 0;
+    uint32_t row_limit = 0;                // only rows < row_limit are needed downstream
+};
+
+// Launch one pass over all 2^D rows (groups = 2^(D-g)) for `count`
+// codewords.
+hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, hipStream_t s);
+size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out);
+
+// Nibble product tables: out[i*64 + q*16 + n] = c[i] * (n << 4q).
+hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s);
+
+// Locator constants for the point set given by the bitmap `known_bits`
+// (bit x of word x/32) over [0, 2^D): the per-row scale of the decode.
+struct AfftLocatorArgs {
+    const uint32_t *known_bits = nullptr;
+    const uint32_t *log_wht = nullptr;  // WHT of log[x], 2^D entries
+    const uint16_t *exp = nullptr;      // 65535 entries
+    uint32_t *work = nullptr;           // 2^D u32
+    uint16_t *out = nullptr;            // 2^D constants
+    uint32_t D = 0;
+};
+hipError_t launch_afft_locator(const AfftLocatorArgs &a, hipStream_t s);
+
+// Row pointer tables of one decode over [0, 2^D): known rows read from
+// source (layout: x < k from data_base + x*L, else repair_base + (x-k)*L),
+// present data copied to out, missing data written to out.
+struct AfftRowsArgs {
+    const uint32_t *known_bits = nullptr;
+    const uint8_t *data_base = nullptr;
+    const uint8_t *repair_base = nullptr;
+    uint8_t *out = nullptr;
+    const uint8_t **in_rows = nullptr;
+    uint8_t **copy_rows = nullptr;
+    uint8_t **out_rows = nullptr;
+    uint32_t k = 0, n = 0, D = 0;
+    uint64_t L = 0;
+};
+hipError_t launch_afft_rows(const AfftRowsArgs &a, hipStream_t s);
+// in_rows[indices[t]] = ptrs[t] for t < count.
+hipError_t launch_afft_scatter_ptrs(const uint16_t *indices, const uint8_t *const *ptrs, uint32_t count,
+                                    const uint8_t **in_rows, hipStream_t s);
+
+// Per-device constant tables.
+struct AfftDeviceTables {
+    DevBuf node_tabs, deriv_tabs, exp;
+};
+hipError_t afft_device_tables(const AfftDeviceTables **out);
+
+// Reed-Solomon codec over GF(2^16) evaluated at the points 0..n-1
+// (points = wire indices, afft/profile.rs:41-46): data symbol i is P(i),
+// repair j is P(k + j), deg P < k.
+struct AfftEngine {
+    scrs_profile prof{};
+    uint32_t k = 0, m = 0, n = 0, D = 0, Dk = 0;  // 2^D >= n, 2^Dk >= k
+    uint64_t N = 0, L = 0;
+    bool fused = false;     // k = 2^Dk, m <= k: interpolate then evaluate the coset [k, 2k)
+    const AfftDeviceTables *dev = nullptr;
+    DevBuf enc_scale;       // systematic locator tables (general encode)
+    DevBuf log_wht;         // decode locator input
+    DevBuf work, work2;
+    DevBuf bits, loc_work, loc_out, dec_scale, in_rows, copy_rows, out_rows;
+    std::vector<uint32_t> h_bits;
+    HostBuf h_bits_pinned;
+    hipEvent_t bits_copied = nullptr;
+    uint64_t enc_cap = 0;   // codewords the encode workspace holds
+    // The workspaces serve one call at a time: calls lock `mu`, and work
+    // enqueued on one stream waits for the previous call's work on another.
+    std::mutex mu;
+    hipEvent_t last = nullptr;
+
+    AfftEngine() = default;
+    AfftEngine(const AfftEngine &) = delete;
+    AfftEngine &operator=(const AfftEngine &) = delete;
+    ~AfftEngine();
+
+    scrs_status init(const scrs_profile &p, bool for_decode);
+    // Order `s` after the previous call (under `mu`); `finish` records it.
+    hipError_t begin(hipStream_t s) { return hipStreamWaitEvent(s, last, 0); }
+    hipError_t finish(hipStream_t s) { return hipEventRecord(last, s); }
+    // `count` codewords back to back: data c at data + c*k*L, repairs at
+    // repairs + c*m*L.
+    scrs_status encode(const uint8_t *data, uint8_t *repairs, uint64_t count, hipStream_t s);
+    // One codeword.  `known` lists the received wire indices (host, exactly
+    // k, distinct, validated).  Payloads from `ptrs` (device array, aligned
+    // with `indices`) or from the data / repair layout.
+    scrs_status decode(const uint16_t *indices, const uint8_t *const *ptrs, const uint8_t *data_base,
+                       const uint8_t *repair_base, uint8_t *out, hipStream_t s);
+};
+
+}  // namespace scrs

@@ -1,0 +1,444 @@
+// afft_kernels.hip - additive FFT over GF(2^16) for gfx950.
+//
+// Butterflies (scrs/src/afft/transform.rs:301-330, tower/payload.rs:58-130):
+//   forward  lo' = lo + f*hi,  hi' = lo' + hi
+//   inverse  hi' = lo + hi,    lo' = lo + f*hi'
+// with f the factor of the butterfly's tree node.  Multiplication by a
+// constant is GF(2)-linear, so c*x = sum over the four nibbles q of x of the
+// table entry c*(n_q << 4q): a constant is a 64-entry u16 table (128 bytes,
+// one dword per LDS bank, so a wave's lookups into one table never
+// conflict).
+//
+// A pass owns the 2^g rows that differ in bits [b_lo, b_lo+g) of the row
+// index, over a 256-byte column tile; each lane owns one dword column.  The
+// g levels run in stages of up to 4 bits: a thread keeps the 16 rows of one
+// stage tile in registers and tiles are exchanged through LDS between
+// stages.  The node tables of the pass are staged in LDS once.
+#include <hip/hip_runtime.h>
+
+#include "afft.hpp"
+#include "afft_math.hpp"
+
+namespace scrs {
+namespace {
+
+constexpr uint32_t kTileDwords = 64;  // 256-byte column tile
+constexpr uint32_t kTab = 64;         // u16 entries per constant table
+
+typedef __attribute__((address_space(4))) const uint64_t *const_u64_ptr;
+template <typename T>
+__device__ __forceinline__ T *load_uniform(T *const *p) {
+    return reinterpret_cast<T *>(*(const_u64_ptr)(p));
+}
+
+__device__ __forceinline__ uint32_t gmul(const uint16_t *t, uint32_t x) {
+    return static_cast<uint32_t>(t[x & 15u]) ^ t[16u | ((x >> 4) & 15u)] ^ t[32u | ((x >> 8) & 15u)] ^
+           t[48u | ((x >> 12) & 15u)];
+}
+
+// Two packed elements times the same constant.
+__device__ __forceinline__ uint32_t gmul2(const uint16_t *t, uint32_t w) {
+    return gmul(t, w & 0xffffu) | (gmul(t, w >> 16) << 16);
+}
+
+__device__ __forceinline__ uint32_t ld_col(const uint8_t *p, uint64_t cb, uint64_t L) {
+    if (((reinterpret_cast<uintptr_t>(p) | L) & 3u) == 0) return *reinterpret_cast<const uint32_t *>(p + cb);
+    uint32_t w = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        if (cb + b < L) w |= static_cast<uint32_t>(p[cb + b]) << (8 * b);
+    return w;
+}
+
+__device__ __forceinline__ void st_col(uint8_t *p, uint64_t cb, uint64_t L, uint32_t w) {
+    if (((reinterpret_cast<uintptr_t>(p) | L) & 3u) == 0) {
+        *reinterpret_cast<uint32_t *>(p + cb) = w;
+        return;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        if (cb + b < L) p[cb + b] = static_cast<uint8_t>(w >> (8 * b));
+}
+
+// Local row of register j in a stage over local bits [s_lo, s_lo + t) for
+// wave `wv` holding R rows: tile u = wv*(R >> t) + (j >> t), i = j & (2^t-1).
+template <int R>
+__device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, uint32_t j) {
+    const uint32_t u = wv * (static_cast<uint32_t>(R) >> t) + (j >> t);
+    const uint32_t i = j & ((1u << t) - 1u);
+    return (u & ((1u << s_lo) - 1u)) | (i << s_lo) | ((u >> s_lo) << (s_lo + t));
+}
+
+template <int R>
+__global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a) {
+    constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
+    extern __shared__ uint32_t smem[];
+    const uint32_t g = a.g, G = 1u << g, b_lo = a.b_lo;
+    const uint32_t gidx = blockIdx.y;
+    const uint32_t base = (gidx & ((1u << b_lo) - 1u)) | ((gidx >> b_lo) << (b_lo + g));
+    if (a.row_limit && (base >> b_lo) > ((a.row_limit - 1u) >> b_lo)) return;
+    const bool inv = a.flags & kAfftInv, der = a.flags & kAfftDeriv, fwd = a.flags & kAfftFwd;
+    uint32_t *data = smem;
+    uint16_t *inv_t = reinterpret_cast<uint16_t *>(smem + G * kTileDwords);
+    uint16_t *fwd_t = inv_t + (inv ? (G - 1) * kTab : 0);
+    uint16_t *der_t = fwd_t + (fwd ? (G - 1) * kTab : 0);
+    uint16_t *sin_t = der_t + (der ? 16 * kTab : 0);
+    uint16_t *sout_t = sin_t + (a.scale_in ? G * kTab : 0);
+    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+
+    // Stage the node tables of this group: level beta (pass bit b_lo+beta)
+    // uses G >> (beta+1) consecutive nodes.
+    auto stage_nodes = [&](uint16_t *dst, uint32_t pbase) {
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
+        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(a.node_tabs);
+        for (uint32_t beta = 0; beta < g; beta++) {
+            const uint32_t B = b_lo + beta;
+            const uint32_t cnt = G >> (beta + 1);
+            const uint32_t first = (1u << (15u - B)) + (pbase >> (B + 1));
+            const uint32_t off = (G - (G >> beta)) * (kTab / 2);
+            for (uint32_t i = tid; i < cnt * (kTab / 2); i += nthr) d32[off + i] = s32[first * (kTab / 2) + i];
+        }
+    };
+    if (inv) stage_nodes(inv_t, a.inv_base + base);
+    if (fwd) stage_nodes(fwd_t, a.fwd_base + base);
+    if (der) {
+        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(a.deriv_tabs);
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(der_t);
+        for (uint32_t i = tid; i < 16 * (kTab / 2); i += nthr) d32[i] = s32[i];
+    }
+    auto stage_rows = [&](uint16_t *dst, const uint16_t *src) {
+        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
+        for (uint32_t i = tid; i < G * (kTab / 2); i += nthr) {
+            const uint32_t r = base | ((i / (kTab / 2)) << b_lo);
+            d32[i] = s32[r * (kTab / 2) + i % (kTab / 2)];
+        }
+    };
+    if (a.scale_in) stage_rows(sin_t, a.scale_in);
+    if (a.scale_out) stage_rows(sout_t, a.scale_out);
+    __syncthreads();
+
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
+    const uint64_t L = a.L;
+    const uint64_t cb = (static_cast<uint64_t>(blockIdx.x) * kTileDwords + lane) * 4u;
+    const bool valid = cb < L;
+    const uint64_t z = blockIdx.z;
+    const uint32_t nst = (g + RB - 1) / RB;
+    uint32_t v[R];
+
+    auto in_ptr = [&](uint32_t r) -> const uint8_t * {
+        if (a.in_rows) return load_uniform(a.in_rows + r);
+        return r < a.in_limit ? a.in_base + z * a.in_cw_stride + static_cast<uint64_t>(r) * L : nullptr;
+    };
+    auto out_ptr = [&](uint32_t r) -> uint8_t * {
+        if (a.out_rows) return load_uniform(a.out_rows + r);
+        return (r >= a.out_first && r < a.out_limit)
+                   ? a.out_base + z * a.out_cw_stride + static_cast<uint64_t>(r - a.out_first) * L
+                   : nullptr;
+    };
+    auto load_global = [&](uint32_t s_lo, uint32_t t) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
+            const uint8_t *src = in_ptr(r);
+            uint32_t w = 0;
+            if (src && valid) w = ld_col(src, cb, L);
+            if (a.copy_rows) {
+                uint8_t *cp = load_uniform(a.copy_rows + r);
+                if (cp && valid) st_col(cp, cb, L, w);
+            }
+            if (a.scale_in) w = gmul2(sin_t + p * kTab, w);
+            v[j] = w;
+        }
+    };
+    auto store_global = [&](uint32_t s_lo, uint32_t t) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
+            uint8_t *dst = out_ptr(r);
+            if (dst && valid) st_col(dst, cb, L, a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
+        }
+    };
+    auto load_lds = [&](uint32_t s_lo, uint32_t t) {
+#pragma unroll
+        for (int j = 0; j < R; j++) v[j] = data[rho<R>(wv, s_lo, t, j) * kTileDwords + lane];
+    };
+    auto store_lds = [&](uint32_t s_lo, uint32_t t) {
+#pragma unroll
+        for (int j = 0; j < R; j++) data[rho<R>(wv, s_lo, t, j) * kTileDwords + lane] = v[j];
+    };
+    auto node_tab = [&](const uint16_t *tabs, uint32_t beta, uint32_t low_row) {
+        return tabs + ((G - (G >> beta)) + (low_row >> (beta + 1))) * kTab;
+    };
+    auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
+#pragma unroll
+        for (int bs = 0; bs < static_cast<int>(RB); bs++) {
+            if (static_cast<uint32_t>(bs) >= t) break;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                if (j & (1 << bs)) continue;
+                const uint16_t *tab = node_tab(inv_t, s_lo + bs, rho<R>(wv, s_lo, t, j));
+                v[j + (1 << bs)] ^= v[j];
+                v[j] ^= gmul2(tab, v[j + (1 << bs)]);
+            }
+        }
+    };
+    auto fwd_stage = [&](uint32_t s_lo, uint32_t t) {
+#pragma unroll
+        for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
+            if (static_cast<uint32_t>(bs) >= t) continue;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                if (j & (1 << bs)) continue;
+                const uint16_t *tab = node_tab(fwd_t, s_lo + bs, rho<R>(wv, s_lo, t, j));
+                v[j] ^= gmul2(tab, v[j + (1 << bs)]);
+                v[j + (1 << bs)] ^= v[j];
+            }
+        }
+    };
+    auto stage_lo = [&](uint32_t k) { return k * RB; };
+    auto stage_t = [&](uint32_t k) { return min(RB, g - k * RB); };
+
+    int cur;
+    if (inv) {
+        cur = 0;
+        load_global(stage_lo(0), stage_t(0));
+        for (uint32_t k = 0; k < nst; k++) {
+            if (k > 0) {
+                store_lds(stage_lo(cur), stage_t(cur));
+                __syncthreads();
+                cur = k;
+                load_lds(stage_lo(cur), stage_t(cur));
+            }
+            inv_stage(stage_lo(k), stage_t(k));
+        }
+    } else {
+        cur = nst - 1;
+        load_global(stage_lo(cur), stage_t(cur));
+    }
+    if (der) {
+        // D[p] = sum over pass bits beta clear in p of d_(b_lo+beta) * C[p | 2^beta],
+        // read from the group's coefficients in LDS.
+        store_lds(stage_lo(cur), stage_t(cur));
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
+            uint32_t acc = 0;
+            for (uint32_t beta = 0; beta < g; beta++)
+                if (!((p >> beta) & 1u))
+                    acc ^= gmul2(der_t + (b_lo + beta) * kTab, data[(p | (1u << beta)) * kTileDwords + lane]);
+            if (a.aux_base && valid) {
+                const uint32_t r = base | (p << b_lo);
+                acc ^= ld_col(a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L, cb, L);
+            }
+            v[j] = acc;
+        }
+        __syncthreads();
+    }
+    if (fwd) {
+        for (int k = static_cast<int>(nst) - 1; k >= 0; k--) {
+            if (cur != k) {
+                store_lds(stage_lo(cur), stage_t(cur));
+                __syncthreads();
+                cur = k;
+                load_lds(stage_lo(cur), stage_t(cur));
+            }
+            fwd_stage(stage_lo(k), stage_t(k));
+        }
+    }
+    store_global(stage_lo(cur), stage_t(cur));
+}
+
+template <int R>
+hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
+    const uint32_t G = 1u << a.g;
+    const dim3 grid(static_cast<uint32_t>((a.L + kTileDwords * 4 - 1) / (kTileDwords * 4)), 1u << (D - a.g), count);
+    const dim3 block((G / R) * 64);
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&afft_pass_kernel<R>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(afft_pass_kernel<R>, grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+// ---- constant tables -------------------------------------------------------
+
+__device__ __forceinline__ uint32_t xtime8(uint32_t x) { return ((x << 1) ^ ((x & 0x80u) ? 0x1Bu : 0u)) & 0xffu; }
+__device__ __forceinline__ uint32_t xtime16(uint32_t v) { return xtime8(v & 0xffu) | (xtime8(v >> 8) << 8); }
+// v * u with u^2 = u + 0x20: (a + b u) u = 0x20 b + (a + b) u.
+__device__ __forceinline__ uint32_t times_u(uint32_t v) {
+    const uint32_t a = v & 0xffu, b = v >> 8;
+    uint32_t lo = b;
+#pragma unroll
+    for (int i = 0; i < 5; i++) lo = xtime8(lo);
+    return lo | ((a ^ b) << 8);
+}
+
+__global__ __launch_bounds__(256) void afft_tables_kernel(const uint16_t *c, uint32_t count, uint16_t *out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    uint32_t basis[16];
+    basis[0] = c[i];
+    for (int b = 1; b < 8; b++) basis[b] = xtime16(basis[b - 1]);
+    basis[8] = times_u(c[i]);
+    for (int b = 9; b < 16; b++) basis[b] = xtime16(basis[b - 1]);
+    uint32_t *o = reinterpret_cast<uint32_t *>(out + static_cast<uint64_t>(i) * kTab);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int n = 0; n < 16; n += 2) {
+            uint32_t e0 = 0, e1 = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                if ((n >> b) & 1) e0 ^= basis[4 * q + b];
+                if (((n + 1) >> b) & 1) e1 ^= basis[4 * q + b];
+            }
+            o[(q * 16 + n) / 2] = e0 | (e1 << 16);
+        }
+}
+
+// ---- locator (afft/decoder.rs:672-771) -------------------------------------
+
+__device__ __forceinline__ uint32_t addmod(uint32_t x, uint32_t y) {
+    const uint32_t s = x + y;
+    return s >= kAfftOrder ? s - kAfftOrder : s;
+}
+__device__ __forceinline__ uint32_t submod(uint32_t x, uint32_t y) { return x >= y ? x - y : x + kAfftOrder - y; }
+__device__ __forceinline__ uint32_t mulmod(uint32_t x, uint32_t y) {
+    const uint32_t p = x * y;  // x, y < 65535: p < 2^32
+    uint32_t t = (p & 0xffffu) + (p >> 16);
+    t = (t & 0xffffu) + (t >> 16);
+    return t >= kAfftOrder ? t - kAfftOrder : t;
+}
+
+enum LocMode : uint32_t {
+    kLocInit = 1,      // start from the unknown-point indicator
+    kLocMulWht = 2,    // multiply by the WHT of log and transform again
+    kLocFinal = 4,     // scale by 2^-D and exponentiate
+    kLocHiLayout = 8,  // 16 columns x 2^H rows of stride 1024
+};
+
+// Block-local Walsh-Hadamard transform mod 65535 over LDS index bits
+// [lo, lo + bits).
+__device__ void lds_wht(uint32_t *s, uint32_t lo, uint32_t bits) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = lo; b < lo + bits; b++) {
+        __syncthreads();
+        const uint32_t h = 1u << b;
+        if (!(tid & h) && (tid | h) < blockDim.x) {
+            const uint32_t x = s[tid], y = s[tid | h];
+            s[tid] = addmod(x, y);
+            s[tid | h] = submod(x, y);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void afft_locator_kernel(AfftLocatorArgs a, uint32_t mode, uint32_t wht_lo,
+                                                            uint32_t wht_bits, uint32_t inv_n) {
+    __shared__ uint32_t s[1024];
+    const uint32_t tid = threadIdx.x;
+    uint32_t x;
+    if (mode & kLocHiLayout) x = blockIdx.x * 16u + (tid & 15u) + ((tid >> 4) << 10);
+    else x = blockIdx.x * blockDim.x + tid;
+    const bool known = (a.known_bits[x >> 5] >> (x & 31u)) & 1u;
+    s[tid] = (mode & kLocInit) ? (known ? 0u : 1u) : a.work[x];
+    lds_wht(s, wht_lo, wht_bits);
+    if (mode & kLocMulWht) {
+        s[tid] = mulmod(s[tid], a.log_wht[x]);
+        lds_wht(s, wht_lo, wht_bits);
+    }
+    if (mode & kLocFinal) {
+        const uint32_t e = mulmod(s[tid], inv_n);
+        a.out[x] = known ? a.exp[e] : a.exp[e ? kAfftOrder - e : 0];
+    } else {
+        a.work[x] = s[tid];
+    }
+}
+
+__global__ __launch_bounds__(256) void afft_rows_kernel(AfftRowsArgs a) {
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= (1u << a.D)) return;
+    const bool known = x < a.n && ((a.known_bits[x >> 5] >> (x & 31u)) & 1u);
+    const uint8_t *src = nullptr;
+    if (known && a.data_base) src = x < a.k ? a.data_base + x * a.L : a.repair_base + (x - a.k) * a.L;
+    a.in_rows[x] = src;
+    a.copy_rows[x] = (known && x < a.k) ? a.out + x * a.L : nullptr;
+    a.out_rows[x] = (!known && x < a.k) ? a.out + x * a.L : nullptr;
+}
+
+__global__ __launch_bounds__(256) void afft_scatter_kernel(const uint16_t *idx, const uint8_t *const *ptrs,
+                                                           uint32_t count, const uint8_t **in_rows) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < count) in_rows[idx[t]] = ptrs[t];
+}
+
+}  // namespace
+
+size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out) {
+    const size_t G = size_t{1} << g, tab = kTab * 2;
+    size_t b = G * kTileDwords * 4;
+    if (flags & kAfftInv) b += (G - 1) * tab;
+    if (flags & kAfftFwd) b += (G - 1) * tab;
+    if (flags & kAfftDeriv) b += 16 * tab;
+    if (scale_in) b += G * tab;
+    if (scale_out) b += G * tab;
+    return b;
+}
+
+hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, hipStream_t s) {
+    if (a.g == 0 || a.g > 8 || a.b_lo + a.g > D || D > kAfftMaxLog || count == 0 || count > 65535 || a.L == 0)
+        return hipErrorInvalidValue;
+    const size_t lds = afft_pass_lds_bytes(a.g, a.flags, a.scale_in != nullptr, a.scale_out != nullptr);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    switch (a.g) {
+        case 1: return launch_pass_r<2>(a, D, count, lds, s);
+        case 2: return launch_pass_r<4>(a, D, count, lds, s);
+        case 3: return launch_pass_r<8>(a, D, count, lds, s);
+        default: return launch_pass_r<16>(a, D, count, lds, s);
+    }
+}
+
+hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(afft_tables_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_afft_locator(const AfftLocatorArgs &a, hipStream_t s) {
+    const uint32_t D = a.D;
+    if (D == 0 || D > kAfftMaxLog) return hipErrorInvalidValue;
+    uint32_t inv_n = 1;
+    for (uint32_t i = 0; i < D; i++) inv_n = static_cast<uint32_t>((static_cast<uint64_t>(inv_n) * 32768u) % kAfftOrder);
+    if (D <= 10) {
+        hipLaunchKernelGGL(afft_locator_kernel, dim3(1), dim3(1u << D), 0, s, a, kLocInit | kLocMulWht | kLocFinal,
+                           0u, D, inv_n);
+        return hipGetLastError();
+    }
+    const uint32_t H = D - 10;
+    hipLaunchKernelGGL(afft_locator_kernel, dim3(1u << H), dim3(1024), 0, s, a, kLocInit, 0u, 10u, inv_n);
+    hipLaunchKernelGGL(afft_locator_kernel, dim3(1024 / 16), dim3(16u << H), 0, s, a, kLocMulWht | kLocHiLayout, 4u,
+                       H, inv_n);
+    hipLaunchKernelGGL(afft_locator_kernel, dim3(1u << H), dim3(1024), 0, s, a, kLocFinal, 0u, 10u, inv_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_afft_rows(const AfftRowsArgs &a, hipStream_t s) {
+    const uint32_t n = 1u << a.D;
+    hipLaunchKernelGGL(afft_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_afft_scatter_ptrs(const uint16_t *indices, const uint8_t *const *ptrs, uint32_t count,
+                                    const uint8_t **in_rows, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(afft_scatter_kernel, dim3((count + 255) / 256), dim3(256), 0, s, indices, ptrs, count,
+                       in_rows);
+    return hipGetLastError();
+}
+
+}  // namespace scrs

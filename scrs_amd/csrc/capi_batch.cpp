@@ -116,7 +116,7 @@ extern "C" scrs_status scrs_batch_encoder_create(const scrs_profile *profile, sc
     enc->own.L = p.symbol_len;
     hipError_t err = hipStreamCreateWithFlags(&enc->stream, hipStreamNonBlocking);
     if (err == hipSuccess && p.engine == SCRS_ENGINE_AFFT) {
-        s = scrs_afft_encoder_init_impl(enc);
+        s = scrs_afft_engine_create(p, false, &enc->afft);
         if (s.code != SCRS_OK) {
             scrs_batch_encoder_destroy(enc);
             return s;
@@ -143,7 +143,7 @@ extern "C" void scrs_batch_encoder_destroy(scrs_batch_encoder *enc) {
         (void)hipStreamSynchronize(enc->stream);
         (void)hipStreamDestroy(enc->stream);
     }
-    if (enc->afft) scrs_afft_encoder_release_impl(enc);
+    if (enc->afft) scrs_afft_engine_destroy(enc->afft);
     delete enc;
 }
 
@@ -215,14 +215,8 @@ extern "C" scrs_status scrs_encode_into_with(const scrs_batch_encoder *enc, cons
     // (selector.rs:516-527): GF(256) codecs take the unit scratch of any
     // GF(256) geometry; AFFT takes an AFFT scratch.
     if (scratch_kind(enc->prof.engine) != scratch_kind(scratch->engine)) return st(SCRS_ENCODE_SCRATCH_MISMATCH);
-    if (enc->prof.engine == SCRS_ENGINE_AFFT &&
-        (scratch->k != enc->prof.k || scratch->m != enc->prof.m || scratch->L != enc->prof.symbol_len)) {
-        // afft::SystematicEncoder::encode_into_with checks lengths, then the
-        // scratch geometry (afft/encoder.rs:250-270).
-        scrs_status s = check_encode_lens(enc->prof, data_len, repairs_len);
-        if (s.code != SCRS_OK) return s;
-        return st(SCRS_ENCODE_SCRATCH_MISMATCH);
-    }
+    // An AFFT scratch is a workspace that is resized on use, so one from
+    // another AFFT geometry is accepted (afft/encoder.rs:250-272).
     scrs_status s = check_encode_lens(enc->prof, data_len, repairs_len);
     if (s.code != SCRS_OK) return s;
     if ((!data && data_len) || (!repairs && repairs_len)) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
@@ -237,7 +231,7 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
     if (!data_dev) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     if (!repairs_dev) return st(SCRS_ERR_INVALID_ARGUMENT, 3);
     if (enc->prof.engine == SCRS_ENGINE_AFFT)
-        return scrs_afft_encode_batch_dev_impl(enc, data_dev, repairs_dev, count, stream);
+        return scrs_afft_encode(enc->afft, data_dev, repairs_dev, count, static_cast<hipStream_t>(stream));
     const uint64_t k = enc->prof.k, m = enc->prof.m, L = enc->prof.symbol_len;
     const bool vec16 = aligned16(data_dev) && aligned16(repairs_dev) && L % 16 == 0;
     RowsTermsArgs a;
@@ -279,7 +273,7 @@ extern "C" scrs_status scrs_batch_decoder_create(const scrs_profile *profile, sc
         scrs_batch_decoder_destroy(dec);
         return construction_error(err, p.engine);
     }
-    s = p.engine == SCRS_ENGINE_AFFT ? scrs_afft_batch_decoder_init_impl(dec) : dec->eng.init(p);
+    s = p.engine == SCRS_ENGINE_AFFT ? scrs_afft_engine_create(p, true, &dec->afft) : dec->eng.init(p);
     if (s.code != SCRS_OK) {
         scrs_batch_decoder_destroy(dec);
         return s;
@@ -295,7 +289,7 @@ extern "C" void scrs_batch_decoder_destroy(scrs_batch_decoder *dec) {
         (void)hipStreamSynchronize(dec->stream);
         (void)hipStreamDestroy(dec->stream);
     }
-    if (dec->afft) scrs_afft_batch_decoder_release_impl(dec);
+    if (dec->afft) scrs_afft_engine_destroy(dec->afft);
     delete dec;
 }
 
@@ -378,14 +372,16 @@ static scrs_status decode_host(scrs_batch_decoder *dec, const uint64_t *indices,
     if ((err = hipMemcpyAsync(sc.d_payload.ptr, sc.h_payload.ptr, k * L, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (err = hipMemcpyAsync(sc.d_ptrs.ptr, ptrs.data(), k * sizeof(void *), hipMemcpyHostToDevice, s)) !=
             hipSuccess ||
-        (err = dec->eng.clear_error(s)) != hipSuccess)
+        (!dec->afft && (err = dec->eng.clear_error(s)) != hipSuccess))
         return st(SCRS_ERR_DEVICE, err);
-    scrs_status r = dec->eng.enqueue(idx16.data(), sc.d_ptrs.as<const uint8_t *const>(), nullptr, nullptr,
-                                     sc.d_out.as<uint8_t>(), 1, s, L % 16 == 0);
+    scrs_status r = dec->afft ? scrs_afft_decode(dec->afft, idx16.data(), sc.d_ptrs.as<const uint8_t *const>(),
+                                                 nullptr, nullptr, sc.d_out.as<uint8_t>(), 1, s)
+                              : dec->eng.enqueue(idx16.data(), sc.d_ptrs.as<const uint8_t *const>(), nullptr,
+                                                 nullptr, sc.d_out.as<uint8_t>(), 1, s, L % 16 == 0);
     if (r.code != SCRS_OK) return r;
     uint32_t flag = 0;
     if ((err = hipMemcpyAsync(sc.h_out.ptr, sc.d_out.ptr, k * L, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-        (err = dec->eng.read_error(&flag, s)) != hipSuccess)
+        (err = dec->afft ? hipStreamSynchronize(s) : dec->eng.read_error(&flag, s)) != hipSuccess)
         return st(SCRS_ERR_DEVICE, err);
     if (flag) return st(SCRS_DECODE_INSUFFICIENT_RANK, k - 1, k);
     std::memcpy(out, sc.h_out.ptr, k * L);
@@ -396,8 +392,6 @@ extern "C" scrs_status scrs_decode_into(scrs_batch_decoder *dec, const uint64_t 
                                         const uint8_t *const *payloads, const uint64_t *payload_lens, uint64_t count,
                                         uint8_t *out, uint64_t out_len) {
     if (!dec) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
-    if (dec->prof.engine == SCRS_ENGINE_AFFT)
-        return scrs_afft_decode_into_impl(dec, indices, payloads, payload_lens, count, out, out_len, nullptr);
     return decode_host(dec, indices, payloads, payload_lens, count, out, out_len, dec->own, false);
 }
 
@@ -409,8 +403,6 @@ extern "C" scrs_status scrs_decode_into_with(scrs_batch_decoder *dec, const uint
     if (!scratch) return st(SCRS_ERR_INVALID_ARGUMENT, 8);
     // Variant check of the type-erased layer comes first (selector.rs:330-357).
     if (scratch_kind(dec->prof.engine) != scratch_kind(scratch->engine)) return st(SCRS_DECODE_SCRATCH_MISMATCH);
-    if (dec->prof.engine == SCRS_ENGINE_AFFT)
-        return scrs_afft_decode_into_impl(dec, indices, payloads, payload_lens, count, out, out_len, scratch);
     return decode_host(dec, indices, payloads, payload_lens, count, out, out_len, *scratch, true);
 }
 
@@ -426,9 +418,11 @@ static scrs_status decode_batch(scrs_batch_decoder *dec, const uint16_t *indices
         scrs_status s = check_indices_u16(indices + c * k, k, n, seen);
         if (s.code != SCRS_OK) return s;
     }
-    if (dec->prof.engine == SCRS_ENGINE_AFFT)
-        return scrs_afft_decode_batch_impl(dec, indices, payload_ptrs_dev, data_dev, repairs_dev, out_dev, count,
-                                           stream);
+    if (dec->prof.engine == SCRS_ENGINE_AFFT) {
+        DeviceGuard g(dec->device);
+        return scrs_afft_decode(dec->afft, indices, payload_ptrs_dev, data_dev, repairs_dev, out_dev, count,
+                                static_cast<hipStream_t>(stream));
+    }
     DeviceGuard g(dec->device);
     // Pointer-table payloads may be unaligned: the 16-byte path is taken
     // only for the layout form, whose alignment is known here.

@@ -36,7 +36,6 @@ extern "C" scrs_status scrs_decoder_create(const scrs_profile *profile, scrs_dec
     scrs_profile p;
     scrs_status s = scrs_profile_resolve(profile->engine, profile->k, profile->m, profile->symbol_len, &p);
     if (s.code != SCRS_OK) return s;
-    if (p.engine == SCRS_ENGINE_AFFT) return st(SCRS_CONFIG_UNSUPPORTED_MODE, p.engine);
     auto *dec = new scrs_decoder();
     dec->prof = p;
     (void)hipGetDevice(&dec->device);
@@ -52,7 +51,7 @@ extern "C" scrs_status scrs_decoder_create(const scrs_profile *profile, scrs_dec
         if (err == hipErrorOutOfMemory) return st(SCRS_CONFIG_TOO_MANY_SYMBOLS, engine_capacity(p.engine));
         return st(SCRS_ERR_DEVICE, err);
     }
-    s = dec->eng.init(p);
+    s = p.engine == SCRS_ENGINE_AFFT ? scrs_afft_engine_create(p, true, &dec->afft) : dec->eng.init(p);
     if (s.code != SCRS_OK) {
         scrs_decoder_destroy(dec);
         return s;
@@ -68,6 +67,7 @@ extern "C" void scrs_decoder_destroy(scrs_decoder *dec) {
         (void)hipStreamSynchronize(dec->stream);
         (void)hipStreamDestroy(dec->stream);
     }
+    if (dec->afft) scrs_afft_engine_destroy(dec->afft);
     delete dec;
 }
 
@@ -207,6 +207,7 @@ static scrs_status finalize_dev(scrs_decoder *dec, uint8_t *out_dev, hipStream_t
     if (err == hipSuccess) err = hipStreamWaitEvent(s, ev, 0);
     (void)hipEventDestroy(ev);
     if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    if (dec->afft) return scrs_afft_decode(dec->afft, idx.data(), nullptr, data_base, repair_base, out_dev, 1, s);
     const bool vec16 = L % 16 == 0 && (reinterpret_cast<uintptr_t>(out_dev) & 15u) == 0;
     const bool cacheable = cache && engine_is_gf256(dec->prof.engine);
     if (cacheable) {
@@ -273,14 +274,15 @@ static scrs_status finalize_host(scrs_decoder *dec, uint8_t *out, uint64_t out_l
     const uint64_t bytes = dec->prof.k * dec->prof.symbol_len;
     hipError_t err;
     if ((err = dec->d_out.ensure(bytes)) != hipSuccess || (err = dec->h_out.ensure(bytes)) != hipSuccess ||
-        (err = dec->eng.clear_error(dec->stream)) != hipSuccess)
+        (!dec->afft && (err = dec->eng.clear_error(dec->stream)) != hipSuccess))
         return st(SCRS_ERR_DEVICE, err);
     s = finalize_dev(dec, dec->d_out.as<uint8_t>(), dec->stream, scratch);
     if (s.code != SCRS_OK) return s;
     uint32_t flag = 0;
     if ((err = hipMemcpyAsync(dec->h_out.ptr, dec->d_out.ptr, bytes, hipMemcpyDeviceToHost, dec->stream)) !=
             hipSuccess ||
-        (err = dec->eng.read_error(&flag, dec->stream)) != hipSuccess)
+        (err = dec->afft ? hipStreamSynchronize(dec->stream) : dec->eng.read_error(&flag, dec->stream)) !=
+            hipSuccess)
         return st(SCRS_ERR_DEVICE, err);
     if (flag) return st(SCRS_DECODE_INSUFFICIENT_RANK, dec->distinct - 1, dec->prof.k);
     std::memcpy(out, dec->h_out.ptr, bytes);
