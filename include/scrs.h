@@ -48,6 +48,8 @@ typedef struct scrs_decode_scratch scrs_decode_scratch;
 typedef struct scrs_decoder scrs_decoder;
 typedef struct scrs_decoder_scratch scrs_decoder_scratch;
 typedef struct scrs_incr_encoder scrs_incr_encoder;
+typedef struct scrs_batch_engine scrs_batch_engine;
+typedef struct scrs_server scrs_server;
 
 /* ---- Library ---------------------------------------------------------- */
 
@@ -250,6 +252,90 @@ scrs_status scrs_incr_reset(scrs_incr_encoder *enc);
  * of the Tower engine. */
 scrs_status scrs_incr_encode_batch_dev(const scrs_incr_encoder *enc, const uint8_t *data_dev,
                                        uint8_t *repairs_dev, uint64_t count, void *stream);
+
+/* ---- Fused batches of uneven codewords (no reference counterpart) ------- */
+
+typedef enum scrs_job_op { SCRS_JOB_ENCODE = 0, SCRS_JOB_DECODE = 1 } scrs_job_op;
+
+/* One codeword's work.  Payload buffers are device memory owned by the
+ * caller.  Encode: src = k*symbol_len data bytes, dst = m*symbol_len repairs
+ * (Tower encodes as feeding all k data symbols).  Decode: indices = k
+ * received wire indices and payloads = their k device pointers (both host
+ * arrays, read during the call), dst = k*symbol_len data bytes. */
+typedef struct scrs_job {
+    int32_t op; /* scrs_job_op */
+    scrs_profile profile;
+    const uint8_t *src;
+    uint8_t *dst;
+    const uint16_t *indices;
+    const uint8_t *const *payloads;
+} scrs_job;
+
+typedef struct scrs_batch_config {
+    uint32_t tile_bytes;  /* column tile per work unit: 1024, 2048, 4096; 0 = autotune */
+    uint32_t cw_per_unit; /* codewords of one encode geometry per unit: 1, 2, 4, 8; 0 = autotune */
+    uint32_t use_graph;   /* replay the batch launches from a captured hipGraph */
+    uint64_t cache_bytes; /* device bytes for resident tables and recipes; 0 = 1 GiB */
+} scrs_batch_config;
+
+typedef struct scrs_batch_stats {
+    uint64_t batches, jobs;
+    uint64_t recipe_hits, recipe_misses; /* decode reconstruction recipes */
+    uint64_t table_hits, table_misses;   /* encode product tables */
+    uint64_t cache_bytes;
+    uint32_t tile_bytes, cw_per_unit;    /* configuration of the last batch */
+    uint32_t tuned_signatures;
+} scrs_batch_stats;
+
+/* A batch engine keeps generator tables and decode recipes resident on the
+ * device and runs a whole batch - GF(256) and Tower jobs of any k, m and
+ * symbol_len - as one persistent launch chain (optionally a hipGraph).
+ * Checks per job follow the codec calls: Profile::resolve, then for decode
+ * IndexOutOfRange -> DuplicateIndex per index; AFFT is UnsupportedMode. */
+scrs_status scrs_batch_engine_create(const scrs_batch_config *cfg, scrs_batch_engine **out);
+void scrs_batch_engine_destroy(scrs_batch_engine *eng);
+/* Validates every job first (on failure *failed_job = its index and nothing
+ * is enqueued), then enqueues the batch on `stream`.  With autotuning, the
+ * first batch of a new shape signature runs once per candidate
+ * configuration, synchronously, and the fastest is kept. */
+scrs_status scrs_batch_engine_submit(scrs_batch_engine *eng, const scrs_job *jobs, uint64_t count,
+                                     void *stream, uint64_t *failed_job);
+scrs_status scrs_batch_engine_stats(const scrs_batch_engine *eng, scrs_batch_stats *out);
+
+/* ---- Serving loop (no reference counterpart) ---------------------------- */
+
+typedef struct scrs_server_config {
+    int32_t batched;         /* 1: continuous batches through a batch engine; 0: one codec call per job */
+    uint32_t max_batch_jobs; /* jobs per batch at most; 0 = 4096 */
+    uint64_t hbm_budget;     /* payload bytes admitted at once; 0 = 8 GiB */
+    scrs_batch_config batch; /* batch engine configuration (batched mode) */
+} scrs_server_config;
+
+typedef struct scrs_completion {
+    uint64_t tag;
+    scrs_status status;
+    uint64_t t_submit_ns, t_start_ns, t_done_ns; /* scrs_now_ns() clock */
+} scrs_completion;
+
+/* A server owns a stream and a dispatcher thread.  Jobs are admitted in
+ * arrival order while their payload footprint fits the budget; in batched
+ * mode everything admitted since the last launch forms the next batch,
+ * queued behind the one on the device, so arrivals join while earlier work
+ * runs.  Completion times are taken when the dispatcher observes the
+ * batch's (or job's) event. */
+scrs_status scrs_server_create(const scrs_server_config *cfg, scrs_server **out);
+void scrs_server_destroy(scrs_server *srv);
+/* Queue one job (copied, including its index and payload arrays). */
+scrs_status scrs_server_submit(scrs_server *srv, const scrs_job *job, uint64_t tag);
+/* Move up to `cap` completions into `out`; wait != 0 blocks until at least
+ * one is available or nothing is outstanding. */
+scrs_status scrs_server_poll(scrs_server *srv, scrs_completion *out, uint64_t cap, uint64_t *n, int32_t wait);
+/* Submit jobs[i] at arrival_ns[i] after the call starts (a native thread
+ * paces the arrivals) and wait for all of them; completions[i] belongs to
+ * jobs[i] (tag = i). */
+scrs_status scrs_server_replay(scrs_server *srv, const scrs_job *jobs, const uint64_t *arrival_ns,
+                               uint64_t count, scrs_completion *completions);
+uint64_t scrs_now_ns(void);
 
 /* ---- Utilities (no reference counterpart) ----------------------------- */
 

@@ -6,6 +6,9 @@ reference's ``Profile`` selection and codec traits.
 """
 from .codec import (  # noqa: F401
     BatchDecoder,
+    BatchEngine,
+    Job,
+    Server,
     BatchEncoder,
     Decoder,
     DecoderScratch,
@@ -24,5 +27,6 @@ from .codec import (  # noqa: F401
     decoder,
     incremental_encoder,
     fill_random_dev,
+    now_ns,
     recommended_gf16_engine,
 )

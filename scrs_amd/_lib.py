@@ -23,6 +23,33 @@ class PushOutcomeStruct(C.Structure):
     _fields_ = [("kind", C.c_int32), ("rank", C.c_uint64), ("received", C.c_uint64)]
 
 
+class JobStruct(C.Structure):
+    _fields_ = [("op", C.c_int32), ("profile", ProfileStruct), ("src", C.c_void_p), ("dst", C.c_void_p),
+                ("indices", C.POINTER(C.c_uint16)), ("payloads", C.POINTER(C.c_void_p))]
+
+
+class BatchConfigStruct(C.Structure):
+    _fields_ = [("tile_bytes", C.c_uint32), ("cw_per_unit", C.c_uint32), ("use_graph", C.c_uint32),
+                ("cache_bytes", C.c_uint64)]
+
+
+class BatchStatsStruct(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("jobs", C.c_uint64), ("recipe_hits", C.c_uint64),
+                ("recipe_misses", C.c_uint64), ("table_hits", C.c_uint64), ("table_misses", C.c_uint64),
+                ("cache_bytes", C.c_uint64), ("tile_bytes", C.c_uint32), ("cw_per_unit", C.c_uint32),
+                ("tuned_signatures", C.c_uint32)]
+
+
+class ServerConfigStruct(C.Structure):
+    _fields_ = [("batched", C.c_int32), ("max_batch_jobs", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("batch", BatchConfigStruct)]
+
+
+class CompletionStruct(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", Status), ("t_submit_ns", C.c_uint64), ("t_start_ns", C.c_uint64),
+                ("t_done_ns", C.c_uint64)]
+
+
 u8p = C.POINTER(C.c_uint8)
 u16p = C.POINTER(C.c_uint16)
 u64p = C.POINTER(C.c_uint64)
@@ -84,6 +111,18 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_incr_fed_count": (C.c_uint64, [H]),
     "scrs_incr_reset": (Status, [H]),
     "scrs_incr_encode_batch_dev": (Status, [H, vp, vp, C.c_uint64, vp]),
+    # fused uneven batches
+    "scrs_batch_engine_create": (Status, [C.POINTER(BatchConfigStruct), HP]),
+    "scrs_batch_engine_destroy": (None, [H]),
+    "scrs_batch_engine_submit": (Status, [H, C.POINTER(JobStruct), C.c_uint64, vp, u64p]),
+    "scrs_batch_engine_stats": (Status, [H, C.POINTER(BatchStatsStruct)]),
+    # serving loop
+    "scrs_server_create": (Status, [C.POINTER(ServerConfigStruct), HP]),
+    "scrs_server_destroy": (None, [H]),
+    "scrs_server_submit": (Status, [H, C.POINTER(JobStruct), C.c_uint64]),
+    "scrs_server_poll": (Status, [H, C.POINTER(CompletionStruct), C.c_uint64, u64p, C.c_int32]),
+    "scrs_server_replay": (Status, [H, C.POINTER(JobStruct), u64p, C.c_uint64, C.POINTER(CompletionStruct)]),
+    "scrs_now_ns": (C.c_uint64, []),
     # utilities
     "scrs_fill_random_dev": (Status, [vp, C.c_uint64, C.c_uint64, C.c_uint64, vp]),
 }

@@ -564,6 +564,129 @@ def incremental_encoder(profile: Profile) -> IncrementalEncoder:
     return IncrementalEncoder(profile)
 
 
+# ---------------------------------------------------------------------------
+# Fused uneven batches and the serving loop
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class Job:
+    """One codeword's work for a BatchEngine or Server (device payloads).
+
+    Encode: ``src`` = k*symbol_len data bytes, ``dst`` = m*symbol_len repairs.
+    Decode: ``indices`` = k received wire indices, ``payloads`` = their device
+    addresses, ``dst`` = k*symbol_len data bytes.
+    """
+
+    op: str
+    profile: Profile
+    src: object = None
+    dst: object = None
+    indices: Sequence[int] | None = None
+    payloads: Sequence[int] | None = None
+
+    @staticmethod
+    def encode(profile: Profile, src_dev, dst_dev) -> "Job":
+        return Job("encode", profile, src_dev, dst_dev)
+
+    @staticmethod
+    def decode(profile: Profile, indices: Sequence[int], payloads: Sequence[int], dst_dev) -> "Job":
+        return Job("decode", profile, None, dst_dev, list(indices), list(payloads))
+
+
+def _job_array(jobs: Sequence[Job]):
+    arr = (_lib.JobStruct * max(len(jobs), 1))()
+    keep = []
+    for i, j in enumerate(jobs):
+        s = arr[i]
+        s.op = 0 if j.op == "encode" else 1
+        s.profile = j.profile._struct()
+        s.src = _dev_ptr(j.src) if j.src is not None else None
+        s.dst = _dev_ptr(j.dst) if j.dst is not None else None
+        if j.op == "decode":
+            idx = np.ascontiguousarray(j.indices, dtype=np.uint16)
+            pay = (C.c_void_p * max(len(j.payloads), 1))(*[int(x) for x in j.payloads])
+            keep += [idx, pay]
+            s.indices = idx.ctypes.data_as(_lib.u16p)
+            s.payloads = C.cast(pay, C.POINTER(C.c_void_p))
+    return arr, keep
+
+
+def _batch_config(tile_bytes: int, cw_per_unit: int, use_graph: bool, cache_bytes: int) -> _lib.BatchConfigStruct:
+    return _lib.BatchConfigStruct(tile_bytes, cw_per_unit, 1 if use_graph else 0, cache_bytes)
+
+
+class BatchEngine(_Handle):
+    """Runs a batch of encode / decode jobs of uneven geometry as one fused,
+    persistent launch chain, with resident tables and decode recipes."""
+
+    _destroy_name = "scrs_batch_engine_destroy"
+
+    def __init__(self, tile_bytes: int = 0, cw_per_unit: int = 0, use_graph: bool = True, cache_bytes: int = 0):
+        h = C.c_void_p()
+        cfg = _batch_config(tile_bytes, cw_per_unit, use_graph, cache_bytes)
+        check(_lib.load().scrs_batch_engine_create(C.byref(cfg), C.byref(h)))
+        super().__init__(h.value)
+
+    def submit(self, jobs: Sequence[Job], stream=None) -> None:
+        arr, _keep = _job_array(jobs)
+        failed = C.c_uint64(0)
+        st = _lib.load().scrs_batch_engine_submit(self._h, arr, len(jobs), _stream_ptr(stream), C.byref(failed))
+        if st.code != 0:
+            err = error_from_status(st.code, st.a, st.b)
+            err.job = int(failed.value)
+            raise err
+
+    def stats(self) -> dict:
+        s = _lib.BatchStatsStruct()
+        check(_lib.load().scrs_batch_engine_stats(self._h, C.byref(s)))
+        return {name: int(getattr(s, name)) for name, _ in s._fields_}
+
+
+class Server(_Handle):
+    """Serving loop: admission by payload footprint, continuous batching
+    (``batched=True``) or one codec call per job."""
+
+    _destroy_name = "scrs_server_destroy"
+
+    def __init__(self, batched: bool = True, max_batch_jobs: int = 0, hbm_budget: int = 0, tile_bytes: int = 0,
+                 cw_per_unit: int = 0, use_graph: bool = True, cache_bytes: int = 0):
+        h = C.c_void_p()
+        cfg = _lib.ServerConfigStruct(1 if batched else 0, max_batch_jobs, hbm_budget,
+                                      _batch_config(tile_bytes, cw_per_unit, use_graph, cache_bytes))
+        check(_lib.load().scrs_server_create(C.byref(cfg), C.byref(h)))
+        super().__init__(h.value)
+
+    def submit(self, job: Job, tag: int) -> None:
+        arr, _keep = _job_array([job])
+        check(_lib.load().scrs_server_submit(self._h, arr, tag))
+
+    def poll(self, cap: int = 1024, wait: bool = False) -> list[dict]:
+        buf = (_lib.CompletionStruct * cap)()
+        n = C.c_uint64(0)
+        check(_lib.load().scrs_server_poll(self._h, buf, cap, C.byref(n), 1 if wait else 0))
+        return [{"tag": int(c.tag), "code": int(c.status.code), "t_submit_ns": int(c.t_submit_ns),
+                 "t_start_ns": int(c.t_start_ns), "t_done_ns": int(c.t_done_ns)} for c in buf[:n.value]]
+
+    def replay(self, jobs: Sequence[Job], arrival_ns: Sequence[int]) -> np.ndarray:
+        """Submit jobs[i] at arrival_ns[i] after the start and wait for all;
+        returns per-job (code, t_submit_ns, t_start_ns, t_done_ns)."""
+        arr, _keep = _job_array(jobs)
+        arrivals = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        out = (_lib.CompletionStruct * max(len(jobs), 1))()
+        check(_lib.load().scrs_server_replay(self._h, arr, arrivals.ctypes.data_as(_lib.u64p), len(jobs), out))
+        res = np.zeros(len(jobs), dtype=[("code", "i4"), ("t_submit_ns", "u8"), ("t_start_ns", "u8"),
+                                         ("t_done_ns", "u8")])
+        for i in range(len(jobs)):
+            c = out[i]
+            res[i] = (c.status.code, c.t_submit_ns, c.t_start_ns, c.t_done_ns)
+        return res
+
+
+def now_ns() -> int:
+    return int(_lib.load().scrs_now_ns())
+
+
 def fill_random_dev(dst_dev, length: int, seed: int, byte_offset: int = 0, stream=None) -> None:
     """Seeded splitmix64 counter-stream bytes written on the device."""
     check(_lib.load().scrs_fill_random_dev(_dev_ptr(dst_dev), length, seed, byte_offset, _stream_ptr(stream)))

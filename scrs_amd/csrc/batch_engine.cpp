@@ -1,0 +1,566 @@
+// batch_engine.cpp - planner, resident caches, graph replay and autotuner
+// of the fused uneven batch, and its C ABI.
+#include "batch_engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#include "engine.hpp"
+#include "gf_device.hpp"
+#include "matrices.hpp"
+#include "status.hpp"
+
+using namespace scrs;
+
+struct scrs_batch_engine {
+    BatchEngine eng;
+};
+
+namespace scrs {
+
+namespace {
+
+constexpr uint32_t kSolveGrid = 256;
+constexpr uint16_t kDataSlot = 0xFFFF;
+
+uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t{15}; }
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+uint32_t log2_floor(uint64_t x) {
+    uint32_t d = 0;
+    while (x > 1) {
+        x >>= 1;
+        d++;
+    }
+    return d;
+}
+int coord_slot(int32_t engine) {
+    return engine == SCRS_ENGINE_STANDARD_CAUCHY ? 0 : (engine == SCRS_ENGINE_GOOD_CAUCHY ? 1 : 2);
+}
+
+}  // namespace
+
+struct BatchEngine::Plan {
+    std::vector<BatchRange> ranges;
+    std::vector<BatchSolve> solves;
+    std::vector<uint64_t> ptrs;   // pointer arrays, device addresses as integers
+    std::vector<uint16_t> u16s;   // received / slot / missing arrays of the solves
+    // Offsets of each range's and solve's arrays, patched into device
+    // addresses at upload.
+    struct RangeRef {
+        uint64_t src = 0, dst = 0, copy = 0;
+        bool has_copy = false;
+    };
+    struct SolveRef {
+        uint64_t idx = 0, slot = 0, missing = 0, work = 0, fused = 0;
+    };
+    std::vector<RangeRef> rref;
+    std::vector<SolveRef> sref;
+    uint64_t work_u32 = 0, fused_u16 = 0;
+    uint32_t units = 0;
+};
+
+BatchEngine::~BatchEngine() {
+    DeviceGuard g(device_);
+    for (Slot &s : slots_) {
+        if (s.done) {
+            (void)hipEventSynchronize(s.done);
+            (void)hipEventDestroy(s.done);
+        }
+        for (auto &kv : s.graphs) (void)hipGraphExecDestroy(kv.second.second);
+    }
+    if (capture_) (void)hipStreamDestroy(capture_);
+    for (CacheEntry *e : lru_) delete e;
+    for (CacheEntry *e : retired_) delete e;
+}
+
+scrs_status BatchEngine::init(const scrs_batch_config &cfg) {
+    cfg_ = cfg;
+    if (cfg_.cache_bytes == 0) cfg_.cache_bytes = 1ull << 30;
+    if (cfg_.tile_bytes != 0 && cfg_.tile_bytes != 1024 && cfg_.tile_bytes != 2048 && cfg_.tile_bytes != 4096)
+        return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    if (cfg_.cw_per_unit != 0 && cfg_.cw_per_unit != 1 && cfg_.cw_per_unit != 2 && cfg_.cw_per_unit != 4 &&
+        cfg_.cw_per_unit != 8)
+        return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    (void)hipGetDevice(&device_);
+    hipError_t err = hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking);
+    // Coordinates of every wire index for each Cauchy engine, at capacity.
+    const int32_t engines[3] = {SCRS_ENGINE_STANDARD_CAUCHY, SCRS_ENGINE_GOOD_CAUCHY, SCRS_ENGINE_TOWER};
+    for (int i = 0; i < 3 && err == hipSuccess; i++) {
+        const scrs_profile p{engines[i], 1, engine_capacity(engines[i]) - 1, 2};
+        const std::vector<uint16_t> c = cauchy_coordinates(p);
+        if ((err = coords_[i].ensure(c.size() * 2)) == hipSuccess)
+            err = hipMemcpy(coords_[i].ptr, c.data(), c.size() * 2, hipMemcpyHostToDevice);
+    }
+    uint8_t t[768];
+    gf8_exp_log(t, t + 512);
+    if (err == hipSuccess && (err = gf8_.ensure(768)) == hipSuccess)
+        err = hipMemcpy(gf8_.ptr, t, 768, hipMemcpyHostToDevice);
+    for (Slot &s : slots_) {
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (err == hipSuccess) err = s.counter.ensure(64);
+    }
+    return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
+}
+
+scrs_status BatchEngine::validate(const scrs_job &j) const {
+    scrs_profile p;
+    scrs_status s = scrs_profile_resolve(j.profile.engine, j.profile.k, j.profile.m, j.profile.symbol_len, &p);
+    if (s.code != SCRS_OK) return s;
+    if (p.engine == SCRS_ENGINE_AFFT) return st(SCRS_CONFIG_UNSUPPORTED_MODE, p.engine);
+    if (j.op == SCRS_JOB_ENCODE) {
+        if (!j.src || !j.dst) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+        return ok();
+    }
+    if (j.op != SCRS_JOB_DECODE) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    if (!j.indices || !j.payloads || !j.dst) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    const uint64_t k = p.k, n = p.k + p.m;
+    std::vector<uint64_t> seen((n + 63) / 64, 0);
+    for (uint64_t t = 0; t < k; t++) {
+        const uint64_t i = j.indices[t];
+        if (i >= n) return st(SCRS_DECODE_INDEX_OUT_OF_RANGE, i, n);
+        if (seen[i >> 6] & (1ull << (i & 63))) return st(SCRS_DECODE_DUPLICATE_INDEX, i);
+        seen[i >> 6] |= 1ull << (i & 63);
+        if (!j.payloads[t]) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    }
+    return ok();
+}
+
+void BatchEngine::touch(CacheEntry *e) {
+    e->last_use = seq_;
+    lru_.remove(e);
+    lru_.push_back(e);
+}
+
+void BatchEngine::retire_completed() {
+    for (Slot &s : slots_)
+        if (s.seq && hipEventQuery(s.done) == hipSuccess) completed_ = std::max(completed_, s.seq);
+    auto it = std::remove_if(retired_.begin(), retired_.end(), [&](CacheEntry *e) {
+        if (e->last_use > completed_) return false;
+        delete e;
+        return true;
+    });
+    retired_.erase(it, retired_.end());
+}
+
+void BatchEngine::evict_to_budget() {
+    for (auto it = lru_.begin(); it != lru_.end() && cache_bytes_ > cfg_.cache_bytes;) {
+        CacheEntry *e = *it;
+        if (e->last_use == seq_) {  // read by the batch being planned
+            ++it;
+            continue;
+        }
+        for (auto m = enc_cache_.begin(); m != enc_cache_.end(); ++m)
+            if (m->second == e) {
+                enc_cache_.erase(m);
+                break;
+            }
+        for (auto m = recipe_cache_.begin(); m != recipe_cache_.end(); ++m)
+            if (m->second == e) {
+                recipe_cache_.erase(m);
+                break;
+            }
+        cache_bytes_ -= e->bytes;
+        it = lru_.erase(it);
+        retired_.push_back(e);  // freed once every batch that read it is done
+    }
+}
+
+CacheEntry *BatchEngine::encoder_tables(const scrs_profile &p, hipError_t *err) {
+    const auto key = std::make_tuple(p.engine, p.k, p.m);
+    auto it = enc_cache_.find(key);
+    if (it != enc_cache_.end()) {
+        st_.table_hits++;
+        touch(it->second);
+        return it->second;
+    }
+    st_.table_misses++;
+    std::vector<uint32_t> terms(p.k);
+    for (uint32_t i = 0; i < p.k; i++) terms[i] = i;
+    auto *e = new CacheEntry();
+    const std::vector<uint32_t> tab = coefficient_tables(p, terms, &e->groups);
+    e->rows = static_cast<uint32_t>(p.m);
+    e->bytes = tab.size() * 4;
+    if ((*err = e->buf.ensure(e->bytes)) != hipSuccess ||
+        (*err = hipMemcpy(e->buf.ptr, tab.data(), e->bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        delete e;
+        return nullptr;
+    }
+    enc_cache_[key] = e;
+    cache_bytes_ += e->bytes;
+    lru_.push_back(e);
+    e->last_use = seq_;
+    return e;
+}
+
+scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p) {
+    p = Plan();
+    hipError_t err = hipSuccess;
+    // Encode jobs of one geometry, length and alignment share a range.
+    std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t, bool>, std::vector<uint64_t>> groups;
+    std::vector<uint64_t> decodes;
+    for (uint64_t i = 0; i < count; i++) {
+        const scrs_job &j = jobs[i];
+        if (j.op == SCRS_JOB_ENCODE) {
+            const bool v = j.profile.symbol_len % 16 == 0 && aligned16(j.src) && aligned16(j.dst);
+            groups[std::make_tuple(j.profile.engine, j.profile.k, j.profile.m, j.profile.symbol_len, v)].push_back(i);
+        } else {
+            decodes.push_back(i);
+        }
+    }
+    uint64_t units = 0;
+    auto add_range = [&](BatchRange r, const Plan::RangeRef &ref) -> bool {
+        r.unit_base = static_cast<uint32_t>(units);
+        units += static_cast<uint64_t>((r.cw_count + r.ncw - 1) / r.ncw) * r.groups * r.tiles;
+        p.ranges.push_back(r);
+        p.rref.push_back(ref);
+        return units <= 0xffffffffull;
+    };
+    for (auto &kv : groups) {
+        const scrs_job &j0 = jobs[kv.second.front()];
+        const scrs_profile &pr = j0.profile;
+        CacheEntry *e = encoder_tables(pr, &err);
+        if (!e) return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+        const uint64_t k = pr.k, m = pr.m, L = pr.symbol_len;
+        BatchRange r;
+        r.L = L;
+        r.tables = e->buf.as<uint32_t>();
+        r.T = static_cast<uint32_t>(k);
+        r.rows = static_cast<uint32_t>(m);
+        r.groups = e->groups;
+        r.tiles = static_cast<uint32_t>((L + tile - 1) / tile);
+        r.cw_count = static_cast<uint32_t>(kv.second.size());
+        r.ncw = std::min<uint32_t>(cw, r.cw_count);
+        r.flags = (std::get<4>(kv.first) ? kRangeVec16 : 0u) | (engine_is_gf256(pr.engine) ? 0u : kRangeGf16);
+        Plan::RangeRef ref;
+        ref.src = p.ptrs.size();
+        for (uint64_t ji : kv.second)
+            for (uint64_t t = 0; t < k; t++) p.ptrs.push_back(reinterpret_cast<uint64_t>(jobs[ji].src + t * L));
+        ref.dst = p.ptrs.size();
+        for (uint64_t ji : kv.second)
+            for (uint64_t t = 0; t < m; t++) p.ptrs.push_back(reinterpret_cast<uint64_t>(jobs[ji].dst + t * L));
+        if (!add_range(r, ref)) return st(SCRS_ERR_CAPACITY, 0xffffffffull);
+    }
+    for (uint64_t ji : decodes) {
+        const scrs_job &j = jobs[ji];
+        const scrs_profile &pr = j.profile;
+        const uint64_t k = pr.k, m = pr.m, n = k + m, L = pr.symbol_len;
+        const bool gf16 = !engine_is_gf256(pr.engine);
+        // Terms in ascending index order: the recipe depends on the set only.
+        std::vector<std::pair<uint16_t, const uint8_t *>> rec(k);
+        for (uint64_t t = 0; t < k; t++) rec[t] = {j.indices[t], j.payloads[t]};
+        std::sort(rec.begin(), rec.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        std::vector<uint64_t> key((n + 63) / 64 + 3, 0);
+        bool vec = L % 16 == 0 && aligned16(j.dst);
+        for (const auto &x : rec) {
+            key[x.first >> 6] |= 1ull << (x.first & 63);
+            vec = vec && aligned16(x.second);
+        }
+        std::vector<uint16_t> missing;
+        for (uint64_t i = 0; i < k; i++)
+            if (!((key[i >> 6] >> (i & 63)) & 1)) missing.push_back(static_cast<uint16_t>(i));
+        const uint32_t r = static_cast<uint32_t>(missing.size());
+        key[key.size() - 3] = static_cast<uint64_t>(pr.engine);
+        key[key.size() - 2] = k;
+        key[key.size() - 1] = m;
+        BatchRange br;
+        br.L = L;
+        br.T = static_cast<uint32_t>(k);
+        br.rows = r;
+        br.groups = r ? (r + 15) / 16 : 1;
+        br.tiles = static_cast<uint32_t>((L + tile - 1) / tile);
+        br.cw_count = 1;
+        br.ncw = 1;
+        br.flags = kRangeDecode | (vec ? kRangeVec16 : 0u) | (gf16 ? kRangeGf16 : 0u);
+        if (r) {
+            CacheEntry *e;
+            auto it = recipe_cache_.find(key);
+            if (it != recipe_cache_.end()) {
+                st_.recipe_hits++;
+                e = it->second;
+                touch(e);
+            } else {
+                st_.recipe_misses++;
+                e = new CacheEntry();
+                e->rows = r;
+                e->groups = br.groups;
+                e->bytes = k * br.groups * table_group_bytes(gf16);
+                if ((err = e->buf.ensure(e->bytes)) != hipSuccess) {
+                    delete e;
+                    return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+                }
+                recipe_cache_[key] = e;
+                cache_bytes_ += e->bytes;
+                lru_.push_back(e);
+                e->last_use = seq_;
+                BatchSolve sv;
+                sv.tables = e->buf.as<uint32_t>();
+                sv.k = static_cast<uint32_t>(k);
+                sv.r = r;
+                sv.groups = br.groups;
+                sv.gf16 = gf16;
+                sv.coords = coords_[coord_slot(pr.engine)].as<uint16_t>();
+                Plan::SolveRef sr;
+                sr.idx = p.u16s.size();
+                for (const auto &x : rec) p.u16s.push_back(x.first);
+                sr.slot = p.u16s.size();
+                uint16_t rank = 0;
+                for (const auto &x : rec) p.u16s.push_back(x.first < k ? kDataSlot : rank++);
+                sr.missing = p.u16s.size();
+                for (uint16_t v : missing) p.u16s.push_back(v);
+                sr.work = p.work_u32;
+                p.work_u32 += 4ull * r + k;
+                sr.fused = p.fused_u16;
+                p.fused_u16 += static_cast<uint64_t>(r) * k;
+                p.solves.push_back(sv);
+                p.sref.push_back(sr);
+            }
+            br.tables = e->buf.as<uint32_t>();
+        }
+        Plan::RangeRef ref;
+        ref.src = p.ptrs.size();
+        for (const auto &x : rec) p.ptrs.push_back(reinterpret_cast<uint64_t>(x.second));
+        ref.dst = p.ptrs.size();
+        for (uint16_t v : missing) p.ptrs.push_back(reinterpret_cast<uint64_t>(j.dst + v * L));
+        ref.copy = p.ptrs.size();
+        ref.has_copy = true;
+        for (const auto &x : rec) p.ptrs.push_back(x.first < k ? reinterpret_cast<uint64_t>(j.dst + x.first * L) : 0);
+        if (!add_range(br, ref)) return st(SCRS_ERR_CAPACITY, 0xffffffffull);
+    }
+    p.units = static_cast<uint32_t>(units);
+    return ok();
+}
+
+hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
+    auto it = slot.graphs.find(tile);
+    if (it == slot.graphs.end() || it->second.first != slot.generation) {
+        if (it != slot.graphs.end()) {
+            (void)hipGraphExecDestroy(it->second.second);
+            slot.graphs.erase(it);
+        }
+        const BatchHeader *hdr = slot.dev.as<BatchHeader>();
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipError_t err = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal);
+        if (err != hipSuccess) return err;
+        // The queue counter is cleared by a kernel node: no memset nodes.
+        hipError_t e1 = launch_batch_reset(slot.counter.as<uint32_t>(), capture_);
+        hipError_t e2 = launch_batch_solve(hdr, gf8_.as<uint8_t>(), kSolveGrid, capture_);
+        hipError_t e3 = launch_batch_rows(hdr, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), capture_);
+        err = hipStreamEndCapture(capture_, &graph);
+        if (err == hipSuccess) err = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3);
+        if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (err != hipSuccess) return err;
+        it = slot.graphs.emplace(tile, std::make_pair(slot.generation, exec)).first;
+    }
+    return hipGraphLaunch(it->second.second, s);
+}
+
+hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStream_t s) {
+    const uint64_t off_r = align16(sizeof(BatchHeader));
+    const uint64_t off_s = align16(off_r + p.ranges.size() * sizeof(BatchRange));
+    const uint64_t off_p = align16(off_s + p.solves.size() * sizeof(BatchSolve));
+    const uint64_t off_u = align16(off_p + p.ptrs.size() * 8);
+    const uint64_t total = align16(off_u + p.u16s.size() * 2);
+    const uint64_t sc_work = 0, sc_fused = align16(p.work_u32 * 4);
+    const uint64_t sc_total = std::max<uint64_t>(16, align16(sc_fused + p.fused_u16 * 2));
+    hipError_t err;
+    // The slot's previous batch may still read its staging and arrays.
+    if ((err = hipEventSynchronize(slot.done)) != hipSuccess) return err;
+    if (total > slot.dev.size || sc_total > slot.scratch.size) {
+        const uint64_t want = std::max<uint64_t>(total * 2, 1 << 20), want_sc = std::max<uint64_t>(sc_total * 2, 1 << 20);
+        if (total > slot.dev.size && (err = slot.dev.ensure(want)) != hipSuccess) return err;
+        if (sc_total > slot.scratch.size && (err = slot.scratch.ensure(want_sc)) != hipSuccess) return err;
+        slot.generation++;
+    }
+    if ((err = slot.host.ensure(slot.dev.size)) != hipSuccess) return err;
+    uint8_t *h = slot.host.as<uint8_t>();
+    uint8_t *d = slot.dev.as<uint8_t>();
+    uint8_t *sc = slot.scratch.as<uint8_t>();
+    BatchHeader hdr;
+    hdr.nranges = static_cast<uint32_t>(p.ranges.size());
+    hdr.nunits = p.units;
+    hdr.nsolve = static_cast<uint32_t>(p.solves.size());
+    hdr.ranges = reinterpret_cast<const BatchRange *>(d + off_r);
+    hdr.solves = reinterpret_cast<const BatchSolve *>(d + off_s);
+    std::memcpy(h, &hdr, sizeof(hdr));
+    auto *hr = reinterpret_cast<BatchRange *>(h + off_r);
+    for (size_t i = 0; i < p.ranges.size(); i++) {
+        BatchRange r = p.ranges[i];
+        const Plan::RangeRef &ref = p.rref[i];
+        r.src = reinterpret_cast<const uint8_t *const *>(d + off_p + ref.src * 8);
+        r.dst = reinterpret_cast<uint8_t *const *>(d + off_p + ref.dst * 8);
+        r.copy = ref.has_copy ? reinterpret_cast<uint8_t *const *>(d + off_p + ref.copy * 8) : nullptr;
+        hr[i] = r;
+    }
+    auto *hs = reinterpret_cast<BatchSolve *>(h + off_s);
+    for (size_t i = 0; i < p.solves.size(); i++) {
+        BatchSolve v = p.solves[i];
+        const Plan::SolveRef &ref = p.sref[i];
+        v.idx = reinterpret_cast<const uint16_t *>(d + off_u + ref.idx * 2);
+        v.slot = reinterpret_cast<const uint16_t *>(d + off_u + ref.slot * 2);
+        v.missing = reinterpret_cast<const uint16_t *>(d + off_u + ref.missing * 2);
+        v.work = reinterpret_cast<uint32_t *>(sc + sc_work + ref.work * 4);
+        v.fused = reinterpret_cast<uint16_t *>(sc + sc_fused + ref.fused * 2);
+        hs[i] = v;
+    }
+    if (!p.ptrs.empty()) std::memcpy(h + off_p, p.ptrs.data(), p.ptrs.size() * 8);
+    if (!p.u16s.empty()) std::memcpy(h + off_u, p.u16s.data(), p.u16s.size() * 2);
+    if ((err = hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s)) != hipSuccess) return err;
+    if (cfg_.use_graph) {
+        err = run_graph(slot, tile, s);
+    } else {
+        const BatchHeader *dh = slot.dev.as<BatchHeader>();
+        if ((err = launch_batch_reset(slot.counter.as<uint32_t>(), s)) == hipSuccess &&
+            (err = launch_batch_solve(dh, gf8_.as<uint8_t>(), kSolveGrid, s)) == hipSuccess)
+            err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), s);
+    }
+    if (err != hipSuccess) return err;
+    slot.seq = seq_;
+    return hipEventRecord(slot.done, s);
+}
+
+scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_t s, uint64_t *failed) {
+    std::lock_guard<std::mutex> lock(mu_);
+    for (uint64_t i = 0; i < count; i++) {
+        scrs_status v = validate(jobs[i]);
+        if (v.code != SCRS_OK) {
+            if (failed) *failed = i;
+            return v;
+        }
+    }
+    if (count == 0) return ok();
+    DeviceGuard g(device_);
+    retire_completed();
+    // Autotuner signature: job-count and mean-length buckets, field mix and
+    // whether any encode geometry repeats (codewords per unit matters).
+    uint64_t lsum = 0, enc = 0, gf16 = 0;
+    std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t>, uint32_t> rep;
+    uint32_t max_rep = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        const scrs_job &j = jobs[i];
+        lsum += j.profile.symbol_len;
+        gf16 += engine_is_gf256(j.profile.engine) ? 0 : 1;
+        if (j.op == SCRS_JOB_ENCODE) {
+            enc++;
+            max_rep = std::max(max_rep, ++rep[std::make_tuple(j.profile.engine, j.profile.k, j.profile.m,
+                                                             j.profile.symbol_len)]);
+        }
+    }
+    const std::vector<uint32_t> sig = {log2_floor(count), log2_floor(lsum / count), gf16 ? 1u : 0u,
+                                       static_cast<uint32_t>(4 * enc / (count + 1)), max_rep > 1 ? 1u : 0u};
+    uint32_t tile = cfg_.tile_bytes, cw = cfg_.cw_per_unit;
+    const bool tune = (tile == 0 || cw == 0);
+    auto found = tuned_.find(sig);
+    if (tune && found != tuned_.end()) {
+        if (!tile) tile = found->second.first;
+        if (!cw) cw = found->second.second;
+    }
+    Plan p;
+    scrs_status r;
+    if (tune && found == tuned_.end()) {
+        // Time every candidate on this batch (each run is a full, correct
+        // execution; the first, untimed, fills the resident caches).
+        std::vector<uint32_t> tiles = tile ? std::vector<uint32_t>{tile} : std::vector<uint32_t>{1024, 2048, 4096};
+        std::vector<uint32_t> cws = cw ? std::vector<uint32_t>{cw}
+                                       : (max_rep > 1 ? std::vector<uint32_t>{1, 2, 4, 8} : std::vector<uint32_t>{1});
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        float best = -1;
+        uint32_t bt = tiles[0], bc = cws[0];
+        bool warm = false;
+        for (uint32_t tt : tiles)
+            for (uint32_t cc : cws) {
+                for (int rep_i = warm ? 0 : -1; rep_i < 1; rep_i++) {
+                    seq_++;
+                    r = plan(jobs, count, tt, cc, p);
+                    if (r.code != SCRS_OK) return r;
+                    evict_to_budget();
+                    Slot &slot = slots_[next_slot_];
+                    next_slot_ ^= 1u;
+                    (void)hipEventRecord(e0, s);
+                    hipError_t err = launch(slot, p, tt, s);
+                    if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+                    (void)hipEventRecord(e1, s);
+                    if ((err = hipEventSynchronize(e1)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (rep_i == 0 && (best < 0 || ms < best)) {
+                        best = ms;
+                        bt = tt;
+                        bc = cc;
+                    }
+                }
+                warm = true;
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tuned_[sig] = {bt, bc};
+        st_.tuned_signatures = static_cast<uint32_t>(tuned_.size());
+        st_.tile_bytes = bt;
+        st_.cw_per_unit = bc;
+        st_.batches++;
+        st_.jobs += count;
+        return ok();
+    }
+    if (!tile) tile = 4096;
+    if (!cw) cw = 1;
+    seq_++;
+    r = plan(jobs, count, tile, cw, p);
+    if (r.code != SCRS_OK) return r;
+    evict_to_budget();
+    Slot &slot = slots_[next_slot_];
+    next_slot_ ^= 1u;
+    hipError_t err = launch(slot, p, tile, s);
+    if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    st_.tile_bytes = tile;
+    st_.cw_per_unit = cw;
+    st_.batches++;
+    st_.jobs += count;
+    return ok();
+}
+
+scrs_batch_stats BatchEngine::stats() const {
+    std::lock_guard<std::mutex> lock(mu_);
+    scrs_batch_stats s = st_;
+    s.cache_bytes = cache_bytes_;
+    return s;
+}
+
+}  // namespace scrs
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" scrs_status scrs_batch_engine_create(const scrs_batch_config *cfg, scrs_batch_engine **out) {
+    if (!out) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    *out = nullptr;
+    scrs_batch_config c{};
+    if (cfg) c = *cfg;
+    auto *e = new scrs_batch_engine();
+    scrs_status s = e->eng.init(c);
+    if (s.code != SCRS_OK) {
+        delete e;
+        return s;
+    }
+    *out = e;
+    return ok();
+}
+
+extern "C" void scrs_batch_engine_destroy(scrs_batch_engine *eng) { delete eng; }
+
+extern "C" scrs_status scrs_batch_engine_submit(scrs_batch_engine *eng, const scrs_job *jobs, uint64_t count,
+                                                void *stream, uint64_t *failed_job) {
+    if (!eng) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    if (!jobs && count) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    return eng->eng.submit(jobs, count, static_cast<hipStream_t>(stream), failed_job);
+}
+
+extern "C" scrs_status scrs_batch_engine_stats(const scrs_batch_engine *eng, scrs_batch_stats *out) {
+    if (!eng) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    if (!out) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    *out = eng->eng.stats();
+    return ok();
+}

@@ -1,0 +1,90 @@
+// batch_engine.hpp - planner, resident caches, graph replay and autotuner
+// of the fused uneven batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <deque>
+#include <list>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/scrs.h"
+#include "batch.hpp"
+#include "device.hpp"
+
+namespace scrs {
+
+// Device buffers kept across batches and released only once the last batch
+// that read them has completed.
+struct CacheEntry {
+    DevBuf buf;
+    uint64_t bytes = 0;
+    uint64_t last_use = 0;  // batch sequence number
+    uint32_t rows = 0, groups = 0;
+};
+
+class BatchEngine {
+  public:
+    BatchEngine() = default;
+    BatchEngine(const BatchEngine &) = delete;
+    BatchEngine &operator=(const BatchEngine &) = delete;
+    ~BatchEngine();
+
+    scrs_status init(const scrs_batch_config &cfg);
+    // Validate every job, then enqueue the whole batch on `s`.  Nothing is
+    // enqueued when a job fails validation; *failed gets its index.
+    scrs_status submit(const scrs_job *jobs, uint64_t count, hipStream_t s, uint64_t *failed);
+    scrs_batch_stats stats() const;
+
+  private:
+    struct Slot {
+        HostBuf host;        // staged header, ranges, solves, pointer and index arrays
+        DevBuf dev;          // device copy of the staged bytes
+        DevBuf scratch;      // solve work and fused coefficients
+        DevBuf counter;      // unit queue
+        hipEvent_t done = nullptr;
+        uint64_t seq = 0;
+        uint32_t generation = 0;  // bumps when `dev` / `scratch` move
+        // Graphs captured against this slot's buffers, per tile width.
+        std::map<uint32_t, std::pair<uint32_t, hipGraphExec_t>> graphs;
+    };
+    struct Plan;
+
+    scrs_status validate(const scrs_job &j) const;
+    scrs_status plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p);
+    hipError_t launch(Slot &slot, const Plan &p, uint32_t tile, hipStream_t s);
+    hipError_t run_graph(Slot &slot, uint32_t tile, hipStream_t s);
+    CacheEntry *encoder_tables(const scrs_profile &p, hipError_t *err);
+    void retire_completed();
+    void touch(CacheEntry *e);
+    void evict_to_budget();
+
+    scrs_batch_config cfg_{};
+    int device_ = 0;
+    hipStream_t capture_ = nullptr;
+    DevBuf coords_[3];   // Standard, Good, Tower coordinates of every wire index
+    DevBuf gf8_;         // exp (512) + log (256)
+    Slot slots_[2];
+    uint32_t next_slot_ = 0;
+    uint64_t seq_ = 0;
+
+    // Resident caches, least recently used first.
+    std::map<std::tuple<int32_t, uint64_t, uint64_t>, CacheEntry *> enc_cache_;
+    std::map<std::vector<uint64_t>, CacheEntry *> recipe_cache_;
+    std::list<CacheEntry *> lru_;
+    std::vector<CacheEntry *> retired_;
+    uint64_t cache_bytes_ = 0;
+    std::deque<std::pair<uint64_t, hipEvent_t>> inflight_;
+    uint64_t completed_ = 0;
+
+    // Autotuner: batch signature -> (tile bytes, codewords per unit).
+    std::map<std::vector<uint32_t>, std::pair<uint32_t, uint32_t>> tuned_;
+    mutable std::mutex mu_;
+    scrs_batch_stats st_{};
+};
+
+}  // namespace scrs

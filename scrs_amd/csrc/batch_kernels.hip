@@ -1,0 +1,362 @@
+// batch_kernels.hip - persistent solve and multiply-XOR over an uneven batch.
+//
+// The multiply-XOR of a unit is the table path of the single-geometry
+// kernels (split-nibble product tables staged in LDS, 16-byte column loads),
+// with every source, destination and copy row taken from the range's
+// pointer tables and every size from its descriptor.  When a range's terms
+// fit one LDS chunk, a unit covering several codewords stages the table
+// once for all of them.
+#include <hip/hip_runtime.h>
+
+#include "batch.hpp"
+#include "gf_device.hpp"
+#include "rows_device.hpp"
+
+namespace scrs {
+namespace {
+
+using namespace rowsdev;
+
+constexpr uint32_t kChunk8 = 32;    // GF(2^8) terms per LDS fill (16 KiB at 16 rows)
+constexpr uint32_t kChunk16 = 16;   // GF(2^16) terms per LDS fill (32 KiB at 16 rows)
+constexpr uint32_t kLds = 32768;
+
+// ---- solve ----------------------------------------------------------------
+
+// Closed-form Cauchy solve of one decode job and the expansion of its fused
+// coefficients into product tables; the same factorisation as the
+// single-geometry solve (decoder/cauchy_inverse.rs:29-170).
+template <typename F>
+__device__ void solve_job(const BatchSolve &j, const F &f) {
+    const uint32_t k = j.k, r = j.r, tid = threadIdx.x, nt = blockDim.x;
+    uint32_t *rv = j.work, *cv = rv + r, *rf = cv + r, *cf = rf + r, *q = cf + r;
+    for (uint32_t t = tid; t < k; t += nt) {
+        const uint32_t s = j.slot[t];
+        if (s != 0xFFFFu) rv[s] = j.coords[j.idx[t]];
+    }
+    for (uint32_t i = tid; i < r; i += nt) cv[i] = j.coords[j.missing[i]];
+    __syncthreads();
+    for (uint32_t i = tid; i < r; i += nt) {
+        uint32_t cross = 1, within = 1;
+        const uint32_t x = rv[i];
+        for (uint32_t h = 0; h < r; h++) cross = f.mul(cross, x ^ cv[h]);
+        for (uint32_t l = 0; l < r; l++)
+            if (l != i) within = f.mul(within, x ^ rv[l]);
+        rf[i] = f.mul(cross, f.inv(within));
+        uint32_t c2 = 1, w2 = 1;
+        const uint32_t y = cv[i];
+        for (uint32_t l = 0; l < r; l++) c2 = f.mul(c2, y ^ rv[l]);
+        for (uint32_t h = 0; h < r; h++)
+            if (h != i) w2 = f.mul(w2, y ^ cv[h]);
+        cf[i] = f.mul(c2, f.inv(w2));
+    }
+    for (uint32_t t = tid; t < k; t += nt) {
+        if (j.slot[t] != 0xFFFFu) continue;
+        const uint32_t z = j.coords[j.idx[t]];
+        uint32_t num = 1, den = 1;
+        for (uint32_t h = 0; h < r; h++) num = f.mul(num, z ^ cv[h]);
+        for (uint32_t l = 0; l < r; l++) den = f.mul(den, z ^ rv[l]);
+        q[t] = f.mul(num, f.inv(den));
+    }
+    __syncthreads();
+    for (uint32_t x = tid; x < r * k; x += nt) {
+        const uint32_t jj = x / k, t = x % k;
+        const uint32_t s = j.slot[t];
+        uint32_t v;
+        if (s != 0xFFFFu) v = f.mul(f.mul(rf[s], cf[jj]), f.inv(cv[jj] ^ rv[s]));
+        else v = f.mul(f.mul(cf[jj], q[t]), f.inv(j.coords[j.idx[t]] ^ cv[jj]));
+        j.fused[jj * k + t] = static_cast<uint16_t>(v);
+    }
+    __syncthreads();
+}
+
+__device__ void tables_job8(const BatchSolve &j, const Gf8Tables &t8) {
+    const uint32_t k = j.k, groups = j.groups, r = j.r;
+    uint8_t *tab = reinterpret_cast<uint8_t *>(j.tables);
+    for (uint32_t x = threadIdx.x; x < k * groups * 16; x += blockDim.x) {
+        const uint32_t s = x % 16, g = (x / 16) % groups, t = x / (16 * groups);
+        const uint32_t row = g * 16 + s;
+        const uint32_t fv = row < r ? j.fused[row * k + t] : 0u;
+        uint32_t basis[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) basis[b] = t8.mul(fv, 1u << b);
+        uint8_t *e = tab + (static_cast<uint64_t>(t) * groups + g) * 512 + s;
+#pragma unroll
+        for (int nib = 0; nib < 16; nib++) {
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if ((nib >> b) & 1) { lo ^= basis[b]; hi ^= basis[4 + b]; }
+            e[nib * 16] = static_cast<uint8_t>(lo);
+            e[(16 + nib) * 16] = static_cast<uint8_t>(hi);
+        }
+    }
+}
+
+__device__ void tables_job16(const BatchSolve &j, const FieldGf65536 &f) {
+    const uint32_t k = j.k, groups = j.groups, r = j.r;
+    uint8_t *tab = reinterpret_cast<uint8_t *>(j.tables);
+    for (uint32_t x = threadIdx.x; x < k * groups * 16; x += blockDim.x) {
+        const uint32_t s = x % 16, g = (x / 16) % groups, t = x / (16 * groups);
+        const uint32_t row = g * 16 + s;
+        const uint32_t fv = row < r ? j.fused[row * k + t] : 0u;
+        uint32_t basis[16];
+#pragma unroll
+        for (int b = 0; b < 16; b++) basis[b] = f.mul(fv, 1u << b);
+        uint8_t *e = tab + (static_cast<uint64_t>(t) * groups + g) * 2048 + 2 * s;
+#pragma unroll
+        for (int qd = 0; qd < 4; qd++)
+#pragma unroll
+            for (int nib = 0; nib < 16; nib++) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if ((nib >> b) & 1) v ^= basis[4 * qd + b];
+                uint8_t *p = e + (qd * 16 + nib) * 32;
+                p[0] = static_cast<uint8_t>(v);
+                p[1] = static_cast<uint8_t>(v >> 8);
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr, const uint8_t *gf8) {
+    __shared__ uint8_t exp[512];
+    __shared__ uint8_t log[256];
+    for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) exp[i] = gf8[i];
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) log[i] = gf8[512 + i];
+    __syncthreads();
+    const Gf8Tables t8{exp, log};
+    const uint32_t n = hdr->nsolve;
+    const BatchSolve *solves = hdr->solves;
+    for (uint32_t jx = blockIdx.x; jx < n; jx += gridDim.x) {
+        const BatchSolve j = solves[jx];
+        if (j.gf16) {
+            const FieldGf65536 f{t8};
+            solve_job(j, f);
+            tables_job16(j, f);
+        } else {
+            const FieldGf256 f{t8};
+            solve_job(j, f);
+            tables_job8(j, t8);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void batch_reset_kernel(uint32_t *counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+// ---- multiply-XOR ----------------------------------------------------------
+
+template <bool VEC, bool DEC, int RW, int TPU>
+__device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here, uint64_t p,
+                      uint8_t *lds) {
+    constexpr uint32_t D = RW / 4;
+    const bool active = p < R.L;
+    const bool single = R.T <= kChunk8;
+    const bool do_copy = DEC && rg == 0;
+    for (uint32_t c = cw0; c < cw0 + ncw; c++) {
+        uint32_t acc[16][D];
+#pragma unroll
+        for (int b = 0; b < 16; b++)
+#pragma unroll
+            for (uint32_t d = 0; d < D; d++) acc[b][d] = 0;
+        for (uint32_t t0 = 0; t0 < R.T; t0 += kChunk8) {
+            const uint32_t tc = min(kChunk8, R.T - t0);
+            if (!single || c == cw0) {
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += TPU) {
+                    const uint32_t t = i / (32 * D), rem = i % (32 * D);
+                    const uint32_t n = rem / D, d = rem % D;
+                    reinterpret_cast<uint32_t *>(lds)[i] = R.tables[((t0 + t) * R.groups + rg) * 128 + n * 4 + d];
+                }
+                __syncthreads();
+            }
+            if (!active) continue;
+#pragma unroll 1
+            for (uint32_t t = 0; t < tc; t++) {
+                const uint32_t gt = c * R.T + t0 + t;
+                uint32_t w[4];
+                load16<VEC>(load_uniform(R.src + gt), p, R.L, w);
+                if (DEC && do_copy) {
+                    uint8_t *cp = load_uniform(R.copy + gt);
+                    if (cp) store16<VEC>(cp, p, R.L, w);
+                }
+                fold_term<RW>(acc, lds + t * 32 * RW, w);
+            }
+        }
+        if (!active) continue;
+#pragma unroll
+        for (int r = 0; r < RW; r++) {
+            if (static_cast<uint32_t>(r) >= rows_here) continue;
+            uint8_t *dst = load_uniform(R.dst + c * R.rows + rg * 16 + r);
+            const int d = r >> 2, s = (r & 3) * 8;
+            uint32_t o[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++)
+                o[jj] = ((acc[4 * jj + 0][d] >> s) & 0xffu) | (((acc[4 * jj + 1][d] >> s) & 0xffu) << 8) |
+                        (((acc[4 * jj + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * jj + 3][d] >> s) & 0xffu) << 24);
+            store16<VEC>(dst, p, R.L, o);
+        }
+    }
+}
+
+template <bool VEC, bool DEC, int RW, int TPU>
+__device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here,
+                       uint64_t p, uint8_t *lds) {
+    constexpr int EB = 2 * RW;
+    constexpr uint32_t D = EB / 4;
+    const bool active = p < R.L;
+    const bool single = R.T <= kChunk16;
+    const bool do_copy = DEC && rg == 0;
+    for (uint32_t c = cw0; c < cw0 + ncw; c++) {
+        uint32_t acc[8][D];
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+#pragma unroll
+            for (uint32_t d = 0; d < D; d++) acc[e][d] = 0;
+        for (uint32_t t0 = 0; t0 < R.T; t0 += kChunk16) {
+            const uint32_t tc = min(kChunk16, R.T - t0);
+            if (!single || c == cw0) {
+                __syncthreads();
+                const uint8_t *tb = reinterpret_cast<const uint8_t *>(R.tables);
+                for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += TPU) {
+                    const uint32_t t = i / (64 * D), rem = i % (64 * D);
+                    const uint32_t e = rem / D, d = rem % D;
+                    reinterpret_cast<uint32_t *>(lds)[i] = *reinterpret_cast<const uint32_t *>(
+                        tb + (static_cast<uint64_t>(t0 + t) * R.groups + rg) * 2048 + e * 32 + d * 4);
+                }
+                __syncthreads();
+            }
+            if (!active) continue;
+#pragma unroll 1
+            for (uint32_t t = 0; t < tc; t++) {
+                const uint32_t gt = c * R.T + t0 + t;
+                uint32_t w[4];
+                load16<VEC>(load_uniform(R.src + gt), p, R.L, w);
+                if (DEC && do_copy) {
+                    uint8_t *cp = load_uniform(R.copy + gt);
+                    if (cp) store16<VEC>(cp, p, R.L, w);
+                }
+                fold_term16<EB>(acc, lds + t * 64 * EB, w);
+            }
+        }
+        if (!active) continue;
+#pragma unroll
+        for (int r = 0; r < RW; r++) {
+            if (static_cast<uint32_t>(r) >= rows_here) continue;
+            uint8_t *dst = load_uniform(R.dst + c * R.rows + rg * 16 + r);
+            const int d = r >> 1, sh = (r & 1) * 16;
+            uint32_t o[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++)
+                o[jj] = ((acc[2 * jj][d] >> sh) & 0xffffu) | (((acc[2 * jj + 1][d] >> sh) & 0xffffu) << 16);
+            store16<VEC>(dst, p, R.L, o);
+        }
+    }
+}
+
+// Decode with no missing data row: only the present rows are copied.
+template <bool VEC>
+__device__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint64_t p) {
+    if (p >= R.L) return;
+    for (uint32_t c = cw0; c < cw0 + ncw; c++)
+        for (uint32_t t = 0; t < R.T; t++) {
+            uint8_t *cp = load_uniform(R.copy + c * R.T + t);
+            if (!cp) continue;
+            uint32_t w[4];
+            load16<VEC>(load_uniform(R.src + c * R.T + t), p, R.L, w);
+            store16<VEC>(cp, p, R.L, w);
+        }
+}
+
+template <bool VEC, bool DEC, int TPU>
+__device__ void run_unit(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint64_t p, uint8_t *lds) {
+    const uint32_t base = rg * 16;
+    const uint32_t rows_here = R.rows > base ? min(16u, R.rows - base) : 0u;
+    if (rows_here == 0) {
+        if (DEC) unit_copy<VEC>(R, cw0, ncw, p);
+        return;
+    }
+    if (R.flags & kRangeGf16) {
+        if (rows_here <= 4) unit16<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+        else if (rows_here <= 8) unit16<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+        else unit16<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+    } else {
+        if (rows_here <= 4) unit8<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+        else if (rows_here <= 8) unit8<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+        else unit8<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+    }
+}
+
+template <int TPU>
+__global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr, uint32_t *counter) {
+    __shared__ __align__(16) uint8_t lds[kLds];
+    __shared__ uint32_t s_unit;
+    const uint32_t nunits = hdr->nunits, nranges = hdr->nranges;
+    const BatchRange *ranges = hdr->ranges;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= nunits) break;
+        // Last range whose first unit is <= u.
+        uint32_t lo = 0, hi = nranges - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (ranges[mid].unit_base <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchRange R = ranges[lo];
+        const uint32_t local = u - R.unit_base, per = R.groups * R.tiles;
+        const uint32_t chunk = local / per, rem = local % per;
+        const uint32_t rg = rem / R.tiles, tile = rem % R.tiles;
+        const uint32_t cw0 = chunk * R.ncw;
+        const uint32_t ncw = min(R.ncw, R.cw_count - cw0);
+        const uint64_t p = static_cast<uint64_t>(tile) * (TPU * 16) + threadIdx.x * 16;
+        const bool vec = R.flags & kRangeVec16, dec = R.flags & kRangeDecode;
+        if (vec && dec) run_unit<true, true, TPU>(R, cw0, ncw, rg, p, lds);
+        else if (vec) run_unit<true, false, TPU>(R, cw0, ncw, rg, p, lds);
+        else if (dec) run_unit<false, true, TPU>(R, cw0, ncw, rg, p, lds);
+        else run_unit<false, false, TPU>(R, cw0, ncw, rg, p, lds);
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch_reset(uint32_t *counter, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables);
+    return hipGetLastError();
+}
+
+uint32_t batch_rows_grid(uint32_t tile_bytes) {
+    // 256 CUs; LDS (32 KiB + a word) bounds residency at four workgroups.
+    (void)tile_bytes;
+    return 256 * 4;
+}
+
+hipError_t launch_batch_rows(const BatchHeader *hdr, uint32_t *counter, uint32_t tile_bytes, uint32_t grid,
+                             hipStream_t s) {
+    switch (tile_bytes) {
+        case 1024:
+            hipLaunchKernelGGL(batch_rows_kernel<64>, dim3(grid), dim3(64), 0, s, hdr, counter);
+            break;
+        case 2048:
+            hipLaunchKernelGGL(batch_rows_kernel<128>, dim3(grid), dim3(128), 0, s, hdr, counter);
+            break;
+        case 4096:
+            hipLaunchKernelGGL(batch_rows_kernel<256>, dim3(grid), dim3(256), 0, s, hdr, counter);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace scrs

@@ -1,0 +1,156 @@
+// rows_device.hpp - device building blocks of the table multiply-XOR kernels:
+// 16-byte column loads and stores, LDS product-table entries and the
+// per-term fold of the GF(2^8) and GF(2^16) kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace scrs {
+namespace rowsdev {
+
+template <bool VEC>
+__device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t L, uint32_t (&w)[4]) {
+    if (VEC) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(src + p);
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t d = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint64_t q = p + j * 4 + b;
+                if (q < L) d |= static_cast<uint32_t>(src[q]) << (8 * b);
+            }
+            w[j] = d;
+        }
+    }
+}
+
+template <bool VEC>
+__device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
+    if (VEC) {
+        *reinterpret_cast<uint4 *>(dst + p) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint64_t q = p + j * 4 + b;
+                if (q < L) dst[q] = static_cast<uint8_t>(w[j] >> (8 * b));
+            }
+    }
+}
+
+// Entry of RW bytes at byte offset `off` of the LDS chunk.
+template <int RW> struct Entry;
+template <> struct Entry<16> {
+    uint32_t d[4];
+    __device__ __forceinline__ void load(const uint8_t *lds, uint32_t off) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(lds + off);
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+};
+template <> struct Entry<8> {
+    uint32_t d[2];
+    __device__ __forceinline__ void load(const uint8_t *lds, uint32_t off) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(lds + off);
+        d[0] = v.x; d[1] = v.y;
+    }
+};
+template <> struct Entry<4> {
+    uint32_t d[1];
+    __device__ __forceinline__ void load(const uint8_t *lds, uint32_t off) {
+        d[0] = *reinterpret_cast<const uint32_t *>(lds + off);
+    }
+};
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// Fold one 16-byte source slice into the accumulators.  `tab` points at the
+// term's 32 entries (lo 0..15, hi 16..31) of RW bytes each.
+template <int RW>
+__device__ __forceinline__ void fold_term(uint32_t (&acc)[16][RW / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
+    constexpr int SH = RW == 16 ? 4 : (RW == 8 ? 3 : 2);  // log2(RW)
+    constexpr uint32_t BMASK = 0x0Fu << SH;                 // one entry offset
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        // Rotations place each byte's low (high) nibble times RW into bits
+        // SH..SH+3 of that byte; masking byte b gives the entry offset.
+        const uint32_t lo = __builtin_amdgcn_alignbit(w[j], w[j], 32 - SH);
+        const uint32_t hi = SH == 4 ? w[j] : __builtin_amdgcn_alignbit(w[j], w[j], 4 - SH);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            Entry<RW> el, eh;
+            el.load(tab, (lo >> (8 * b)) & BMASK);
+            eh.load(tab + 16 * RW, (hi >> (8 * b)) & BMASK);
+#pragma unroll
+            for (int d = 0; d < RW / 4; d++) acc[4 * j + b][d] = xor3(acc[4 * j + b][d], el.d[d], eh.d[d]);
+        }
+        // Fold four columns at a time: keeps the lookups in flight bounded
+        // and the register budget at four waves per SIMD.
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
+// Entries of the per-codeword pointer / row-count tables are the same for
+// every lane and the kernel never writes those tables: read them through the
+// constant address space so they become scalar loads into SGPRs.
+typedef __attribute__((address_space(4))) const uint64_t *const_u64_ptr;
+typedef __attribute__((address_space(4))) const uint32_t *const_u32_ptr;
+
+template <typename T>
+__device__ __forceinline__ T *load_uniform(T *const *ptr) {
+    return reinterpret_cast<T *>(*(const_u64_ptr)(ptr));
+}
+__device__ __forceinline__ uint32_t load_uniform(const uint32_t *ptr) { return *(const_u32_ptr)(ptr); }
+
+// ---- GF(2^16): entries of EB = 2 * rows bytes, 64 entries per term ----
+
+template <int EB>
+__device__ __forceinline__ void load_entry(const uint8_t *lds, uint32_t off, uint32_t (&d)[EB / 4]) {
+    if constexpr (EB == 32) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(lds + off);
+        const uint4 b = *reinterpret_cast<const uint4 *>(lds + off + 16);
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    } else if constexpr (EB == 16) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(lds + off);
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+    } else {
+        const uint2 a = *reinterpret_cast<const uint2 *>(lds + off);
+        d[0] = a.x; d[1] = a.y;
+    }
+}
+
+// Fold the 8 elements of a 16-byte slice.  `tab` = the term's 64 entries
+// (nibble position q major, value n minor) of EB bytes.
+template <int EB>
+__device__ __forceinline__ void fold_term16(uint32_t (&acc)[8][EB / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
+    constexpr int SH = EB == 32 ? 5 : (EB == 16 ? 4 : 3);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t x = (w[j] >> (16 * h)) & 0xffffu;
+            uint32_t e0[EB / 4], e1[EB / 4], e2[EB / 4], e3[EB / 4];
+            load_entry<EB>(tab, ((x & 15u) << SH), e0);
+            load_entry<EB>(tab, (16u << SH) + (((x >> 4) & 15u) << SH), e1);
+            load_entry<EB>(tab, (32u << SH) + (((x >> 8) & 15u) << SH), e2);
+            load_entry<EB>(tab, (48u << SH) + ((x >> 12) << SH), e3);
+            uint32_t(&a)[EB / 4] = acc[2 * j + h];
+#pragma unroll
+            for (int d = 0; d < EB / 4; d++) a[d] = xor3(xor3(a[d], e0[d], e1[d]), e2[d], e3[d]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+}  // namespace rowsdev
+}  // namespace scrs
