@@ -35,6 +35,9 @@ struct BatchRange {
     uint32_t flags = 0;
 };
 
+// A table job: either the closed-form solve of one decode recipe (fused
+// coefficients, then tables), or the product tables of an encode geometry
+// (coefficient 1 / (x_t + y_row) straight from the coordinates).
 struct BatchSolve {
     const uint16_t *coords = nullptr;   // engine coordinate of every wire index
     const uint16_t *idx = nullptr;      // k received indices, received order
@@ -44,6 +47,8 @@ struct BatchSolve {
     uint16_t *fused = nullptr;          // r * k
     uint32_t *tables = nullptr;         // k * groups tables
     uint32_t k = 0, r = 0, groups = 0, gf16 = 0;
+    uint32_t encode = 0;                // 1: encode tables for k terms and r = m rows
+    uint32_t pad = 0;
 };
 
 struct BatchHeader {

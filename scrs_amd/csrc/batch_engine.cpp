@@ -22,6 +22,7 @@ namespace {
 
 constexpr uint32_t kSolveGrid = 256;
 constexpr uint16_t kDataSlot = 0xFFFF;
+constexpr uint64_t kTuneMinJobs = 32;
 
 uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t{15}; }
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
@@ -69,6 +70,7 @@ BatchEngine::~BatchEngine() {
         for (auto &kv : s.graphs) (void)hipGraphExecDestroy(kv.second.second);
     }
     if (capture_) (void)hipStreamDestroy(capture_);
+    if (last_done_) (void)hipEventDestroy(last_done_);
     for (CacheEntry *e : lru_) delete e;
     for (CacheEntry *e : retired_) delete e;
 }
@@ -95,6 +97,8 @@ scrs_status BatchEngine::init(const scrs_batch_config &cfg) {
     gf8_exp_log(t, t + 512);
     if (err == hipSuccess && (err = gf8_.ensure(768)) == hipSuccess)
         err = hipMemcpy(gf8_.ptr, t, 768, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = arena_.init(cfg_.cache_bytes);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&last_done_, hipEventDisableTiming);
     for (Slot &s : slots_) {
         if (err == hipSuccess) err = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         if (err == hipSuccess) err = s.counter.ensure(64);
@@ -125,10 +129,46 @@ scrs_status BatchEngine::validate(const scrs_job &j) const {
     return ok();
 }
 
+hipError_t DeviceArena::init(uint64_t bytes) {
+    cap_ = bytes & ~uint64_t{255};
+    hipError_t err = buf_.ensure(cap_);
+    if (err == hipSuccess) free_[0] = cap_;
+    return err;
+}
+
+bool DeviceArena::alloc(uint64_t bytes, uint64_t *off) {
+    bytes = (bytes + 255) & ~uint64_t{255};
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        *off = it->first;
+        const uint64_t rest = it->second - bytes, at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        return true;
+    }
+    return false;
+}
+
+void DeviceArena::release(uint64_t off, uint64_t bytes) {
+    bytes = (bytes + 255) & ~uint64_t{255};
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + bytes == next->first) {
+        bytes += next->second;
+        next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+            prev->second += bytes;
+            return;
+        }
+    }
+    free_[off] = bytes;
+}
+
 void BatchEngine::touch(CacheEntry *e) {
     e->last_use = seq_;
-    lru_.remove(e);
-    lru_.push_back(e);
+    lru_.splice(lru_.end(), lru_, e->lru);
 }
 
 void BatchEngine::retire_completed() {
@@ -136,36 +176,58 @@ void BatchEngine::retire_completed() {
         if (s.seq && hipEventQuery(s.done) == hipSuccess) completed_ = std::max(completed_, s.seq);
     auto it = std::remove_if(retired_.begin(), retired_.end(), [&](CacheEntry *e) {
         if (e->last_use > completed_) return false;
+        arena_.release(e->off, e->bytes);
         delete e;
         return true;
     });
     retired_.erase(it, retired_.end());
 }
 
-void BatchEngine::evict_to_budget() {
-    for (auto it = lru_.begin(); it != lru_.end() && cache_bytes_ > cfg_.cache_bytes;) {
+// Free arena space: evict the least recently used entry the batch being
+// planned does not read, or wait for earlier batches to release evicted
+// entries.  False when nothing can be freed.
+bool BatchEngine::make_room() {
+    for (auto it = lru_.begin(); it != lru_.end(); ++it) {
         CacheEntry *e = *it;
-        if (e->last_use == seq_) {  // read by the batch being planned
-            ++it;
-            continue;
-        }
-        for (auto m = enc_cache_.begin(); m != enc_cache_.end(); ++m)
-            if (m->second == e) {
-                enc_cache_.erase(m);
-                break;
-            }
-        for (auto m = recipe_cache_.begin(); m != recipe_cache_.end(); ++m)
-            if (m->second == e) {
-                recipe_cache_.erase(m);
-                break;
-            }
+        if (e->last_use == seq_) continue;
+        if (e->recipe) recipe_cache_.erase(e->recipe_key);
+        else enc_cache_.erase(e->enc_key);
         cache_bytes_ -= e->bytes;
-        it = lru_.erase(it);
-        retired_.push_back(e);  // freed once every batch that read it is done
+        lru_.erase(it);
+        retired_.push_back(e);
+        retire_completed();
+        return true;
     }
+    if (retired_.empty()) return false;
+    for (Slot &s : slots_) (void)hipEventSynchronize(s.done);
+    retire_completed();
+    return true;
 }
 
-CacheEntry *BatchEngine::encoder_tables(const scrs_profile &p, hipError_t *err) {
+CacheEntry *BatchEngine::new_entry(uint64_t bytes, scrs_status *err) {
+    if (bytes > arena_.capacity()) {
+        *err = st(SCRS_ERR_CAPACITY, arena_.capacity());
+        return nullptr;
+    }
+    uint64_t off = 0;
+    while (!arena_.alloc(bytes, &off)) {
+        if (!make_room()) {
+            *err = st(SCRS_ERR_CAPACITY, arena_.capacity());
+            return nullptr;
+        }
+    }
+    auto *e = new CacheEntry();
+    e->off = off;
+    e->bytes = bytes;
+    e->last_use = seq_;
+    cache_bytes_ += bytes;
+    e->lru = lru_.insert(lru_.end(), e);
+    return e;
+}
+
+// Encode product tables of one geometry; a new geometry's tables are
+// computed on the device by a table job of the batch being planned.
+CacheEntry *BatchEngine::encoder_tables(const scrs_profile &p, Plan &plan, scrs_status *err) {
     const auto key = std::make_tuple(p.engine, p.k, p.m);
     auto it = enc_cache_.find(key);
     if (it != enc_cache_.end()) {
@@ -174,27 +236,29 @@ CacheEntry *BatchEngine::encoder_tables(const scrs_profile &p, hipError_t *err) 
         return it->second;
     }
     st_.table_misses++;
-    std::vector<uint32_t> terms(p.k);
-    for (uint32_t i = 0; i < p.k; i++) terms[i] = i;
-    auto *e = new CacheEntry();
-    const std::vector<uint32_t> tab = coefficient_tables(p, terms, &e->groups);
+    const bool gf16 = !engine_is_gf256(p.engine);
+    const uint32_t groups = static_cast<uint32_t>((p.m + 15) / 16);
+    CacheEntry *e = new_entry(p.k * groups * table_group_bytes(gf16), err);
+    if (!e) return nullptr;
     e->rows = static_cast<uint32_t>(p.m);
-    e->bytes = tab.size() * 4;
-    if ((*err = e->buf.ensure(e->bytes)) != hipSuccess ||
-        (*err = hipMemcpy(e->buf.ptr, tab.data(), e->bytes, hipMemcpyHostToDevice)) != hipSuccess) {
-        delete e;
-        return nullptr;
-    }
+    e->groups = groups;
+    e->enc_key = key;
     enc_cache_[key] = e;
-    cache_bytes_ += e->bytes;
-    lru_.push_back(e);
-    e->last_use = seq_;
+    BatchSolve sv;
+    sv.coords = coords_[coord_slot(p.engine)].as<uint16_t>();
+    sv.tables = entry_ptr(e);
+    sv.k = static_cast<uint32_t>(p.k);
+    sv.r = static_cast<uint32_t>(p.m);
+    sv.groups = groups;
+    sv.gf16 = gf16;
+    sv.encode = 1;
+    plan.solves.push_back(sv);
+    plan.sref.emplace_back();
     return e;
 }
 
 scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p) {
     p = Plan();
-    hipError_t err = hipSuccess;
     // Encode jobs of one geometry, length and alignment share a range.
     std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t, bool>, std::vector<uint64_t>> groups;
     std::vector<uint64_t> decodes;
@@ -218,12 +282,13 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
     for (auto &kv : groups) {
         const scrs_job &j0 = jobs[kv.second.front()];
         const scrs_profile &pr = j0.profile;
-        CacheEntry *e = encoder_tables(pr, &err);
-        if (!e) return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+        scrs_status es = ok();
+        CacheEntry *e = encoder_tables(pr, p, &es);
+        if (!e) return es;
         const uint64_t k = pr.k, m = pr.m, L = pr.symbol_len;
         BatchRange r;
         r.L = L;
-        r.tables = e->buf.as<uint32_t>();
+        r.tables = entry_ptr(e);
         r.T = static_cast<uint32_t>(k);
         r.rows = static_cast<uint32_t>(m);
         r.groups = e->groups;
@@ -280,20 +345,16 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
                 touch(e);
             } else {
                 st_.recipe_misses++;
-                e = new CacheEntry();
+                scrs_status es = ok();
+                e = new_entry(k * br.groups * table_group_bytes(gf16), &es);
+                if (!e) return es;
                 e->rows = r;
                 e->groups = br.groups;
-                e->bytes = k * br.groups * table_group_bytes(gf16);
-                if ((err = e->buf.ensure(e->bytes)) != hipSuccess) {
-                    delete e;
-                    return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
-                }
+                e->recipe = true;
+                e->recipe_key = key;
                 recipe_cache_[key] = e;
-                cache_bytes_ += e->bytes;
-                lru_.push_back(e);
-                e->last_use = seq_;
                 BatchSolve sv;
-                sv.tables = e->buf.as<uint32_t>();
+                sv.tables = entry_ptr(e);
                 sv.k = static_cast<uint32_t>(k);
                 sv.r = r;
                 sv.groups = br.groups;
@@ -314,7 +375,7 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
                 p.solves.push_back(sv);
                 p.sref.push_back(sr);
             }
-            br.tables = e->buf.as<uint32_t>();
+            br.tables = entry_ptr(e);
         }
         Plan::RangeRef ref;
         ref.src = p.ptrs.size();
@@ -397,6 +458,10 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     for (size_t i = 0; i < p.solves.size(); i++) {
         BatchSolve v = p.solves[i];
         const Plan::SolveRef &ref = p.sref[i];
+        if (v.encode) {
+            hs[i] = v;
+            continue;
+        }
         v.idx = reinterpret_cast<const uint16_t *>(d + off_u + ref.idx * 2);
         v.slot = reinterpret_cast<const uint16_t *>(d + off_u + ref.slot * 2);
         v.missing = reinterpret_cast<const uint16_t *>(d + off_u + ref.missing * 2);
@@ -406,6 +471,9 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     }
     if (!p.ptrs.empty()) std::memcpy(h + off_p, p.ptrs.data(), p.ptrs.size() * 8);
     if (!p.u16s.empty()) std::memcpy(h + off_u, p.u16s.data(), p.u16s.size() * 2);
+    // Tables and recipes written by the previous batch may be read here, and
+    // it may have run on another stream.
+    if ((err = hipStreamWaitEvent(s, last_done_, 0)) != hipSuccess) return err;
     if ((err = hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s)) != hipSuccess) return err;
     if (cfg_.use_graph) {
         err = run_graph(slot, tile, s);
@@ -417,7 +485,8 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     }
     if (err != hipSuccess) return err;
     slot.seq = seq_;
-    return hipEventRecord(slot.done, s);
+    if ((err = hipEventRecord(slot.done, s)) != hipSuccess) return err;
+    return hipEventRecord(last_done_, s);
 }
 
 scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_t s, uint64_t *failed) {
@@ -447,7 +516,7 @@ scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_
                                                              j.profile.symbol_len)]);
         }
     }
-    const std::vector<uint32_t> sig = {log2_floor(count), log2_floor(lsum / count), gf16 ? 1u : 0u,
+    const std::vector<uint32_t> sig = {log2_floor(lsum / count), gf16 ? 1u : 0u,
                                        static_cast<uint32_t>(4 * enc / (count + 1)), max_rep > 1 ? 1u : 0u};
     uint32_t tile = cfg_.tile_bytes, cw = cfg_.cw_per_unit;
     const bool tune = (tile == 0 || cw == 0);
@@ -458,7 +527,9 @@ scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_
     }
     Plan p;
     scrs_status r;
-    if (tune && found == tuned_.end()) {
+    // Small batches are not representative: they run with the defaults
+    // until a large batch of their signature has been tuned.
+    if (tune && found == tuned_.end() && count >= kTuneMinJobs) {
         // Time every candidate on this batch (each run is a full, correct
         // execution; the first, untimed, fills the resident caches).
         std::vector<uint32_t> tiles = tile ? std::vector<uint32_t>{tile} : std::vector<uint32_t>{1024, 2048, 4096};
@@ -476,7 +547,6 @@ scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_
                     seq_++;
                     r = plan(jobs, count, tt, cc, p);
                     if (r.code != SCRS_OK) return r;
-                    evict_to_budget();
                     Slot &slot = slots_[next_slot_];
                     next_slot_ ^= 1u;
                     (void)hipEventRecord(e0, s);
@@ -509,7 +579,6 @@ scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_
     seq_++;
     r = plan(jobs, count, tile, cw, p);
     if (r.code != SCRS_OK) return r;
-    evict_to_budget();
     Slot &slot = slots_[next_slot_];
     next_slot_ ^= 1u;
     hipError_t err = launch(slot, p, tile, s);

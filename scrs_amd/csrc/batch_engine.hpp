@@ -18,13 +18,32 @@
 
 namespace scrs {
 
-// Device buffers kept across batches and released only once the last batch
-// that read them has completed.
+// Resident tables live in one device arena: first-fit over a free list with
+// coalescing, so filling a cache entry costs no allocation call.
+class DeviceArena {
+  public:
+    hipError_t init(uint64_t bytes);
+    bool alloc(uint64_t bytes, uint64_t *off);
+    void release(uint64_t off, uint64_t bytes);
+    uint8_t *base() const { return buf_.as<uint8_t>(); }
+    uint64_t capacity() const { return cap_; }
+
+  private:
+    DevBuf buf_;
+    uint64_t cap_ = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> size
+};
+
+// A cache entry is released only once the last batch that read it has
+// completed.
 struct CacheEntry {
-    DevBuf buf;
-    uint64_t bytes = 0;
+    uint64_t off = 0, bytes = 0;
     uint64_t last_use = 0;  // batch sequence number
     uint32_t rows = 0, groups = 0;
+    std::list<CacheEntry *>::iterator lru;
+    bool recipe = false;
+    std::tuple<int32_t, uint64_t, uint64_t> enc_key;
+    std::vector<uint64_t> recipe_key;
 };
 
 class BatchEngine {
@@ -58,10 +77,12 @@ class BatchEngine {
     scrs_status plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p);
     hipError_t launch(Slot &slot, const Plan &p, uint32_t tile, hipStream_t s);
     hipError_t run_graph(Slot &slot, uint32_t tile, hipStream_t s);
-    CacheEntry *encoder_tables(const scrs_profile &p, hipError_t *err);
+    CacheEntry *encoder_tables(const scrs_profile &p, Plan &plan, scrs_status *err);
+    CacheEntry *new_entry(uint64_t bytes, scrs_status *err);
+    bool make_room();
     void retire_completed();
     void touch(CacheEntry *e);
-    void evict_to_budget();
+    uint32_t *entry_ptr(const CacheEntry *e) const { return reinterpret_cast<uint32_t *>(arena_.base() + e->off); }
 
     scrs_batch_config cfg_{};
     int device_ = 0;
@@ -72,6 +93,8 @@ class BatchEngine {
     uint32_t next_slot_ = 0;
     uint64_t seq_ = 0;
 
+    DeviceArena arena_;
+    hipEvent_t last_done_ = nullptr;  // the previous batch, on whatever stream
     // Resident caches, least recently used first.
     std::map<std::tuple<int32_t, uint64_t, uint64_t>, CacheEntry *> enc_cache_;
     std::map<std::vector<uint64_t>, CacheEntry *> recipe_cache_;

@@ -70,13 +70,21 @@ __device__ void solve_job(const BatchSolve &j, const F &f) {
     __syncthreads();
 }
 
+// Coefficient of term t for output row `row` of a table job.
+template <typename F>
+__device__ __forceinline__ uint32_t table_coef(const BatchSolve &j, const F &f, uint32_t t, uint32_t row) {
+    if (row >= j.r) return 0u;
+    if (j.encode) return f.inv(static_cast<uint32_t>(j.coords[t]) ^ j.coords[j.k + row]);
+    return j.fused[row * j.k + t];
+}
+
 __device__ void tables_job8(const BatchSolve &j, const Gf8Tables &t8) {
-    const uint32_t k = j.k, groups = j.groups, r = j.r;
+    const uint32_t k = j.k, groups = j.groups;
+    const FieldGf256 f{t8};
     uint8_t *tab = reinterpret_cast<uint8_t *>(j.tables);
     for (uint32_t x = threadIdx.x; x < k * groups * 16; x += blockDim.x) {
         const uint32_t s = x % 16, g = (x / 16) % groups, t = x / (16 * groups);
-        const uint32_t row = g * 16 + s;
-        const uint32_t fv = row < r ? j.fused[row * k + t] : 0u;
+        const uint32_t fv = table_coef(j, f, t, g * 16 + s);
         uint32_t basis[8];
 #pragma unroll
         for (int b = 0; b < 8; b++) basis[b] = t8.mul(fv, 1u << b);
@@ -94,12 +102,11 @@ __device__ void tables_job8(const BatchSolve &j, const Gf8Tables &t8) {
 }
 
 __device__ void tables_job16(const BatchSolve &j, const FieldGf65536 &f) {
-    const uint32_t k = j.k, groups = j.groups, r = j.r;
+    const uint32_t k = j.k, groups = j.groups;
     uint8_t *tab = reinterpret_cast<uint8_t *>(j.tables);
     for (uint32_t x = threadIdx.x; x < k * groups * 16; x += blockDim.x) {
         const uint32_t s = x % 16, g = (x / 16) % groups, t = x / (16 * groups);
-        const uint32_t row = g * 16 + s;
-        const uint32_t fv = row < r ? j.fused[row * k + t] : 0u;
+        const uint32_t fv = table_coef(j, f, t, g * 16 + s);
         uint32_t basis[16];
 #pragma unroll
         for (int b = 0; b < 16; b++) basis[b] = f.mul(fv, 1u << b);
@@ -132,11 +139,11 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
         const BatchSolve j = solves[jx];
         if (j.gf16) {
             const FieldGf65536 f{t8};
-            solve_job(j, f);
+            if (!j.encode) solve_job(j, f);
             tables_job16(j, f);
         } else {
             const FieldGf256 f{t8};
-            solve_job(j, f);
+            if (!j.encode) solve_job(j, f);
             tables_job8(j, t8);
         }
         __syncthreads();

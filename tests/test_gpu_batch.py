@@ -154,7 +154,7 @@ def test_graph_replays_match_eager(torch_cuda):
 
 def test_autotuner_picks_a_configuration(torch_cuda):
     torch = torch_cuda
-    w = Workload(torch, 7, 30, lens=(512, 4096))
+    w = Workload(torch, 7, 40, lens=(512, 4096))
     eng = BatchEngine(tile_bytes=0, cw_per_unit=0)
     eng.submit(w.jobs)
     torch.cuda.synchronize()
