@@ -229,13 +229,22 @@ def serve_workload(args) -> dict:
         else:
             gaps = np.random.default_rng(0xA55).exponential(1e9 / rate, size=n_req)
             arr = np.cumsum(gaps).astype(np.uint64)
+        before = srv.stats()
         res = srv.replay(prepared, arr)
+        after = srv.stats()
         if (res["code"] != 0).any():
-            raise SystemExit("a serving request failed")
+            bad = np.flatnonzero(res["code"] != 0)
+            raise SystemExit(f"{bad.size} serving requests failed, first {int(bad[0])} with status "
+                             f"{int(res['code'][bad[0]])}")
         lat = (res["t_done_ns"] - res["t_submit_ns"]) / 1e6
         span = (res["t_done_ns"].max() - res["t_submit_ns"].min()) / 1e9
         out = {"gbs": round(total_alg / span / 1e9, 2), "req_per_s": round(n_req / span, 1),
                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+        nb = max(1, after["batches"] - before["batches"])
+        out["batches"] = after["batches"] - before["batches"]
+        out["mean_batch_jobs"] = round((after["jobs"] - before["jobs"]) / nb, 1)
+        out["host_us_per_batch"] = round((after["host_ns"] - before["host_ns"]) / nb / 1e3, 1)
+        out["gpu_us_per_batch"] = round((after["gpu_ns"] - before["gpu_ns"]) / nb / 1e3, 1)
         print(f"serve {'batched' if batched else 'plain'} rate={rate}: {out}", file=sys.stderr, flush=True)
         return out
 

@@ -337,6 +337,14 @@ scrs_status scrs_server_replay(scrs_server *srv, const scrs_job *jobs, const uin
                                uint64_t count, scrs_completion *completions);
 uint64_t scrs_now_ns(void);
 
+typedef struct scrs_server_stats {
+    uint64_t batches, jobs, max_batch_jobs;
+    uint64_t host_ns;           /* dispatcher time spent enqueueing batches */
+    uint64_t gpu_ns, max_gpu_ns; /* device time of the batches (events around them) */
+} scrs_server_stats;
+/* Counters since creation; read after the dispatcher has seen the work. */
+scrs_status scrs_server_get_stats(scrs_server *srv, scrs_server_stats *out);
+
 /* ---- Utilities (no reference counterpart) ----------------------------- */
 
 /* Fill len bytes of device memory with the seeded splitmix64 counter stream

@@ -45,6 +45,11 @@ class ServerConfigStruct(C.Structure):
                 ("batch", BatchConfigStruct)]
 
 
+class ServerStatsStruct(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("jobs", C.c_uint64), ("max_batch_jobs", C.c_uint64),
+                ("host_ns", C.c_uint64), ("gpu_ns", C.c_uint64), ("max_gpu_ns", C.c_uint64)]
+
+
 class CompletionStruct(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("status", Status), ("t_submit_ns", C.c_uint64), ("t_start_ns", C.c_uint64),
                 ("t_done_ns", C.c_uint64)]
@@ -123,6 +128,7 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_server_poll": (Status, [H, C.POINTER(CompletionStruct), C.c_uint64, u64p, C.c_int32]),
     "scrs_server_replay": (Status, [H, C.POINTER(JobStruct), u64p, C.c_uint64, C.POINTER(CompletionStruct)]),
     "scrs_now_ns": (C.c_uint64, []),
+    "scrs_server_get_stats": (Status, [H, C.POINTER(ServerStatsStruct)]),
     # utilities
     "scrs_fill_random_dev": (Status, [vp, C.c_uint64, C.c_uint64, C.c_uint64, vp]),
 }

@@ -668,6 +668,11 @@ class Server(_Handle):
         return [{"tag": int(c.tag), "code": int(c.status.code), "t_submit_ns": int(c.t_submit_ns),
                  "t_start_ns": int(c.t_start_ns), "t_done_ns": int(c.t_done_ns)} for c in buf[:n.value]]
 
+    def stats(self) -> dict:
+        s = _lib.ServerStatsStruct()
+        check(_lib.load().scrs_server_get_stats(self._h, C.byref(s)))
+        return {name: int(getattr(s, name)) for name, _ in s._fields_}
+
     @staticmethod
     def prepare(jobs: Sequence[Job]):
         """Marshal a job list once for repeated replays."""

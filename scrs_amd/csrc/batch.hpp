@@ -33,6 +33,9 @@ struct BatchRange {
     uint32_t cw_count = 0, ncw = 1;       // codewords, codewords per unit
     uint32_t unit_base = 0;               // first unit of the range
     uint32_t flags = 0;
+    // Term segments: with segs > 1 each unit folds seg_terms terms and XORs
+    // its partial rows into destinations zeroed beforehand (16-byte rows).
+    uint32_t segs = 1, seg_terms = 0;
 };
 
 // A table job: either the closed-form solve of one decode recipe (fused
@@ -52,9 +55,10 @@ struct BatchSolve {
 };
 
 struct BatchHeader {
-    uint32_t nranges = 0, nunits = 0, nsolve = 0, pad = 0;
+    uint32_t nranges = 0, nunits = 0, nsolve = 0, nzero = 0;
     const BatchRange *ranges = nullptr;
     const BatchSolve *solves = nullptr;
+    const uint64_t *zero = nullptr;  // nzero (address, bytes) pairs cleared before the multiply-XOR
 };
 
 // Tile widths the multiply-XOR kernel is compiled for (16 bytes per thread).

@@ -23,6 +23,7 @@ namespace {
 constexpr uint32_t kSolveGrid = 256;
 constexpr uint16_t kDataSlot = 0xFFFF;
 constexpr uint64_t kTuneMinJobs = 32;
+constexpr uint64_t kFillUnits = 512;  // two units per CU
 
 uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t{15}; }
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
@@ -56,6 +57,7 @@ struct BatchEngine::Plan {
     };
     std::vector<RangeRef> rref;
     std::vector<SolveRef> sref;
+    std::vector<uint64_t> zero;   // (address, bytes) of rows cleared before term-split ranges
     uint64_t work_u32 = 0, fused_u16 = 0;
     uint32_t units = 0;
 };
@@ -273,6 +275,8 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
     }
     uint64_t units = 0;
     auto add_range = [&](BatchRange r, const Plan::RangeRef &ref) -> bool {
+        r.segs = 1;
+        r.seg_terms = r.T;
         r.unit_base = static_cast<uint32_t>(units);
         units += static_cast<uint64_t>((r.cw_count + r.ncw - 1) / r.ncw) * r.groups * r.tiles;
         p.ranges.push_back(r);
@@ -387,8 +391,46 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
         for (const auto &x : rec) p.ptrs.push_back(x.first < k ? reinterpret_cast<uint64_t>(j.dst + x.first * L) : 0);
         if (!add_range(br, ref)) return st(SCRS_ERR_CAPACITY, 0xffffffffull);
     }
+    split_terms(p, units);
     p.units = static_cast<uint32_t>(units);
     return ok();
+}
+
+// A batch too small to give every CU work splits the term loop of its
+// larger aligned ranges: each segment's unit XORs its partial rows into
+// destinations the table launch clears first.  Latency of a lone large
+// codeword drops by the split factor; a full batch is left as it is.
+void BatchEngine::split_terms(Plan &p, uint64_t &units) {
+    if (units >= kFillUnits || units == 0) return;
+    const uint64_t factor = (kFillUnits + units - 1) / units;
+    bool any = false;
+    for (size_t i = 0; i < p.ranges.size(); i++) {
+        BatchRange &r = p.ranges[i];
+        const uint32_t chunk = (r.flags & kRangeGf16) ? 16u : 32u;
+        if (!(r.flags & kRangeVec16) || r.rows == 0 || r.T <= chunk) continue;
+        const uint32_t max_segs = (r.T + chunk - 1) / chunk;
+        const uint32_t want = static_cast<uint32_t>(std::min<uint64_t>(max_segs, factor));
+        if (want < 2) continue;
+        r.seg_terms = ((r.T + want - 1) / want + chunk - 1) / chunk * chunk;
+        r.segs = (r.T + r.seg_terms - 1) / r.seg_terms;
+        if (r.segs < 2) {
+            r.segs = 1;
+            r.seg_terms = r.T;
+            continue;
+        }
+        any = true;
+        const Plan::RangeRef &ref = p.rref[i];
+        for (uint64_t x = 0; x < static_cast<uint64_t>(r.cw_count) * r.rows; x++) {
+            p.zero.push_back(p.ptrs[ref.dst + x]);
+            p.zero.push_back(r.L);
+        }
+    }
+    if (!any) return;
+    units = 0;
+    for (BatchRange &r : p.ranges) {
+        r.unit_base = static_cast<uint32_t>(units);
+        units += static_cast<uint64_t>((r.cw_count + r.ncw - 1) / r.ncw) * r.groups * r.tiles * r.segs;
+    }
 }
 
 hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
@@ -422,7 +464,8 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     const uint64_t off_s = align16(off_r + p.ranges.size() * sizeof(BatchRange));
     const uint64_t off_p = align16(off_s + p.solves.size() * sizeof(BatchSolve));
     const uint64_t off_u = align16(off_p + p.ptrs.size() * 8);
-    const uint64_t total = align16(off_u + p.u16s.size() * 2);
+    const uint64_t off_z = align16(off_u + p.u16s.size() * 2);
+    const uint64_t total = align16(off_z + p.zero.size() * 8);
     const uint64_t sc_work = 0, sc_fused = align16(p.work_u32 * 4);
     const uint64_t sc_total = std::max<uint64_t>(16, align16(sc_fused + p.fused_u16 * 2));
     hipError_t err;
@@ -444,6 +487,8 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     hdr.nsolve = static_cast<uint32_t>(p.solves.size());
     hdr.ranges = reinterpret_cast<const BatchRange *>(d + off_r);
     hdr.solves = reinterpret_cast<const BatchSolve *>(d + off_s);
+    hdr.nzero = static_cast<uint32_t>(p.zero.size() / 2);
+    hdr.zero = reinterpret_cast<const uint64_t *>(d + off_z);
     std::memcpy(h, &hdr, sizeof(hdr));
     auto *hr = reinterpret_cast<BatchRange *>(h + off_r);
     for (size_t i = 0; i < p.ranges.size(); i++) {
@@ -471,6 +516,7 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     }
     if (!p.ptrs.empty()) std::memcpy(h + off_p, p.ptrs.data(), p.ptrs.size() * 8);
     if (!p.u16s.empty()) std::memcpy(h + off_u, p.u16s.data(), p.u16s.size() * 2);
+    if (!p.zero.empty()) std::memcpy(h + off_z, p.zero.data(), p.zero.size() * 8);
     // Tables and recipes written by the previous batch may be read here, and
     // it may have run on another stream.
     if ((err = hipStreamWaitEvent(s, last_done_, 0)) != hipSuccess) return err;

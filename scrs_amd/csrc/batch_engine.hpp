@@ -75,6 +75,7 @@ class BatchEngine {
 
     scrs_status validate(const scrs_job &j) const;
     scrs_status plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p);
+    void split_terms(Plan &p, uint64_t &units);
     hipError_t launch(Slot &slot, const Plan &p, uint32_t tile, hipStream_t s);
     hipError_t run_graph(Slot &slot, uint32_t tile, hipStream_t s);
     CacheEntry *encoder_tables(const scrs_profile &p, Plan &plan, scrs_status *err);

@@ -148,6 +148,12 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
         }
         __syncthreads();
     }
+    // Destinations of term-split ranges start from zero.
+    for (uint32_t z = blockIdx.x; z < hdr->nzero; z += gridDim.x) {
+        uint4 *dst = reinterpret_cast<uint4 *>(hdr->zero[2 * z]);
+        const uint64_t n16 = hdr->zero[2 * z + 1] / 16;
+        for (uint64_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = make_uint4(0, 0, 0, 0);
+    }
 }
 
 __global__ void batch_reset_kernel(uint32_t *counter) {
@@ -156,21 +162,29 @@ __global__ void batch_reset_kernel(uint32_t *counter) {
 
 // ---- multiply-XOR ----------------------------------------------------------
 
+// XOR a 16-byte row slice into device memory (term-split partial sums).
+__device__ __forceinline__ void xor16(uint8_t *dst, uint64_t p, const uint32_t (&o)[4]) {
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst + p);
+#pragma unroll
+    for (int j = 0; j < 4; j++) atomicXor(d + j, o[j]);
+}
+
 template <bool VEC, bool DEC, int RW, int TPU>
 __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here, uint64_t p,
-                      uint8_t *lds) {
+                      uint32_t tb, uint32_t te, uint8_t *lds) {
     constexpr uint32_t D = RW / 4;
     const bool active = p < R.L;
-    const bool single = R.T <= kChunk8;
+    const bool single = te - tb <= kChunk8;
     const bool do_copy = DEC && rg == 0;
+    const bool split = VEC && R.segs > 1;
     for (uint32_t c = cw0; c < cw0 + ncw; c++) {
         uint32_t acc[16][D];
 #pragma unroll
         for (int b = 0; b < 16; b++)
 #pragma unroll
             for (uint32_t d = 0; d < D; d++) acc[b][d] = 0;
-        for (uint32_t t0 = 0; t0 < R.T; t0 += kChunk8) {
-            const uint32_t tc = min(kChunk8, R.T - t0);
+        for (uint32_t t0 = tb; t0 < te; t0 += kChunk8) {
+            const uint32_t tc = min(kChunk8, te - t0);
             if (!single || c == cw0) {
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += TPU) {
@@ -204,27 +218,29 @@ __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t 
             for (int jj = 0; jj < 4; jj++)
                 o[jj] = ((acc[4 * jj + 0][d] >> s) & 0xffu) | (((acc[4 * jj + 1][d] >> s) & 0xffu) << 8) |
                         (((acc[4 * jj + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * jj + 3][d] >> s) & 0xffu) << 24);
-            store16<VEC>(dst, p, R.L, o);
+            if (split) xor16(dst, p, o);
+            else store16<VEC>(dst, p, R.L, o);
         }
     }
 }
 
 template <bool VEC, bool DEC, int RW, int TPU>
 __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here,
-                       uint64_t p, uint8_t *lds) {
+                       uint64_t p, uint32_t tb, uint32_t te, uint8_t *lds) {
     constexpr int EB = 2 * RW;
     constexpr uint32_t D = EB / 4;
     const bool active = p < R.L;
-    const bool single = R.T <= kChunk16;
+    const bool single = te - tb <= kChunk16;
     const bool do_copy = DEC && rg == 0;
+    const bool split = VEC && R.segs > 1;
     for (uint32_t c = cw0; c < cw0 + ncw; c++) {
         uint32_t acc[8][D];
 #pragma unroll
         for (int e = 0; e < 8; e++)
 #pragma unroll
             for (uint32_t d = 0; d < D; d++) acc[e][d] = 0;
-        for (uint32_t t0 = 0; t0 < R.T; t0 += kChunk16) {
-            const uint32_t tc = min(kChunk16, R.T - t0);
+        for (uint32_t t0 = tb; t0 < te; t0 += kChunk16) {
+            const uint32_t tc = min(kChunk16, te - t0);
             if (!single || c == cw0) {
                 __syncthreads();
                 const uint8_t *tb = reinterpret_cast<const uint8_t *>(R.tables);
@@ -259,17 +275,18 @@ __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t
 #pragma unroll
             for (int jj = 0; jj < 4; jj++)
                 o[jj] = ((acc[2 * jj][d] >> sh) & 0xffffu) | (((acc[2 * jj + 1][d] >> sh) & 0xffffu) << 16);
-            store16<VEC>(dst, p, R.L, o);
+            if (split) xor16(dst, p, o);
+            else store16<VEC>(dst, p, R.L, o);
         }
     }
 }
 
 // Decode with no missing data row: only the present rows are copied.
 template <bool VEC>
-__device__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint64_t p) {
+__device__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint64_t p, uint32_t tb, uint32_t te) {
     if (p >= R.L) return;
     for (uint32_t c = cw0; c < cw0 + ncw; c++)
-        for (uint32_t t = 0; t < R.T; t++) {
+        for (uint32_t t = tb; t < te; t++) {
             uint8_t *cp = load_uniform(R.copy + c * R.T + t);
             if (!cp) continue;
             uint32_t w[4];
@@ -279,21 +296,23 @@ __device__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint6
 }
 
 template <bool VEC, bool DEC, int TPU>
-__device__ void run_unit(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint64_t p, uint8_t *lds) {
+__device__ void run_unit(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t seg, uint64_t p,
+                         uint8_t *lds) {
     const uint32_t base = rg * 16;
     const uint32_t rows_here = R.rows > base ? min(16u, R.rows - base) : 0u;
+    const uint32_t tb = seg * R.seg_terms, te = min(R.T, tb + R.seg_terms);
     if (rows_here == 0) {
-        if (DEC) unit_copy<VEC>(R, cw0, ncw, p);
+        if (DEC) unit_copy<VEC>(R, cw0, ncw, p, tb, te);
         return;
     }
     if (R.flags & kRangeGf16) {
-        if (rows_here <= 4) unit16<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
-        else if (rows_here <= 8) unit16<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
-        else unit16<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+        if (rows_here <= 4) unit16<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
+        else if (rows_here <= 8) unit16<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
+        else unit16<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
     } else {
-        if (rows_here <= 4) unit8<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
-        else if (rows_here <= 8) unit8<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
-        else unit8<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, lds);
+        if (rows_here <= 4) unit8<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
+        else if (rows_here <= 8) unit8<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
+        else unit8<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
     }
 }
 
@@ -316,17 +335,18 @@ __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr,
             else hi = mid - 1;
         }
         const BatchRange R = ranges[lo];
-        const uint32_t local = u - R.unit_base, per = R.groups * R.tiles;
+        const uint32_t local = u - R.unit_base, per = R.groups * R.tiles * R.segs;
         const uint32_t chunk = local / per, rem = local % per;
-        const uint32_t rg = rem / R.tiles, tile = rem % R.tiles;
+        const uint32_t seg = rem % R.segs, rt = rem / R.segs;
+        const uint32_t rg = rt / R.tiles, tile = rt % R.tiles;
         const uint32_t cw0 = chunk * R.ncw;
         const uint32_t ncw = min(R.ncw, R.cw_count - cw0);
         const uint64_t p = static_cast<uint64_t>(tile) * (TPU * 16) + threadIdx.x * 16;
         const bool vec = R.flags & kRangeVec16, dec = R.flags & kRangeDecode;
-        if (vec && dec) run_unit<true, true, TPU>(R, cw0, ncw, rg, p, lds);
-        else if (vec) run_unit<true, false, TPU>(R, cw0, ncw, rg, p, lds);
-        else if (dec) run_unit<false, true, TPU>(R, cw0, ncw, rg, p, lds);
-        else run_unit<false, false, TPU>(R, cw0, ncw, rg, p, lds);
+        if (vec && dec) run_unit<true, true, TPU>(R, cw0, ncw, rg, seg, p, lds);
+        else if (vec) run_unit<true, false, TPU>(R, cw0, ncw, rg, seg, p, lds);
+        else if (dec) run_unit<false, true, TPU>(R, cw0, ncw, rg, seg, p, lds);
+        else run_unit<false, false, TPU>(R, cw0, ncw, rg, seg, p, lds);
         __syncthreads();
     }
 }

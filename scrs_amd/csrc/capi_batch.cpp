@@ -114,8 +114,8 @@ extern "C" scrs_status scrs_batch_encoder_create(const scrs_profile *profile, sc
     enc->own.k = p.k;
     enc->own.m = p.m;
     enc->own.L = p.symbol_len;
-    hipError_t err = hipStreamCreateWithFlags(&enc->stream, hipStreamNonBlocking);
-    if (err == hipSuccess && p.engine == SCRS_ENGINE_AFFT) {
+    hipError_t err = hipSuccess;
+    if (p.engine == SCRS_ENGINE_AFFT) {
         s = scrs_afft_engine_create(p, false, &enc->afft);
         if (s.code != SCRS_OK) {
             scrs_batch_encoder_destroy(enc);
@@ -183,6 +183,10 @@ static scrs_status encode_host(const scrs_batch_encoder *enc, const uint8_t *dat
         (err = sc.h_rep.ensure(m * L)) != hipSuccess || (err = sc.d_rep.ensure(m * L)) != hipSuccess)
         return st(SCRS_ERR_DEVICE, err);
     std::memcpy(sc.h_data.ptr, data, k * L);
+    auto *mut = const_cast<scrs_batch_encoder *>(enc);
+    std::call_once(mut->stream_once,
+                   [mut] { mut->stream_err = hipStreamCreateWithFlags(&mut->stream, hipStreamNonBlocking); });
+    if (mut->stream_err != hipSuccess) return st(SCRS_ERR_DEVICE, mut->stream_err);
     hipStream_t s = enc->stream;
     if ((err = hipMemcpyAsync(sc.d_data.ptr, sc.h_data.ptr, k * L, hipMemcpyHostToDevice, s)) != hipSuccess)
         return st(SCRS_ERR_DEVICE, err);
@@ -268,11 +272,6 @@ extern "C" scrs_status scrs_batch_decoder_create(const scrs_profile *profile, sc
     dec->own.k = p.k;
     dec->own.m = p.m;
     dec->own.L = p.symbol_len;
-    hipError_t err = hipStreamCreateWithFlags(&dec->stream, hipStreamNonBlocking);
-    if (err != hipSuccess) {
-        scrs_batch_decoder_destroy(dec);
-        return construction_error(err, p.engine);
-    }
     s = p.engine == SCRS_ENGINE_AFFT ? scrs_afft_engine_create(p, true, &dec->afft) : dec->eng.init(p);
     if (s.code != SCRS_OK) {
         scrs_batch_decoder_destroy(dec);
@@ -361,6 +360,8 @@ static scrs_status decode_host(scrs_batch_decoder *dec, const uint64_t *indices,
     if (!out) return st(SCRS_ERR_INVALID_ARGUMENT, 6);
     DeviceGuard g(dec->device);
     hipError_t err;
+    if (!dec->stream && (err = hipStreamCreateWithFlags(&dec->stream, hipStreamNonBlocking)) != hipSuccess)
+        return st(SCRS_ERR_DEVICE, err);
     if ((err = sc.h_payload.ensure(k * L)) != hipSuccess || (err = sc.d_payload.ensure(k * L)) != hipSuccess ||
         (err = sc.h_out.ensure(k * L)) != hipSuccess || (err = sc.d_out.ensure(k * L)) != hipSuccess ||
         (err = sc.d_ptrs.ensure(k * sizeof(void *))) != hipSuccess)

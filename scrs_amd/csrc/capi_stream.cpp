@@ -334,21 +334,13 @@ extern "C" scrs_status scrs_incr_encoder_create(const scrs_profile *profile, scr
     enc->prof = p;
     enc->gf16 = p.engine == SCRS_ENGINE_TOWER;
     (void)hipGetDevice(&enc->device);
-    const uint64_t m = p.m, L = p.symbol_len;
     enc->fed.assign(p.k, 0);
     std::vector<uint32_t> terms(p.k);
     for (uint32_t i = 0; i < p.k; i++) terms[i] = i;
     const std::vector<uint32_t> tables = coefficient_tables(p, terms, &enc->groups);
     enc->group_bytes = table_group_bytes(enc->gf16);
-    hipError_t err = hipStreamCreateWithFlags(&enc->stream, hipStreamNonBlocking);
-    if (err == hipSuccess) err = enc->d_tables.ensure(tables.size() * 4);
+    hipError_t err = enc->d_tables.ensure(tables.size() * 4);
     if (err == hipSuccess) err = hipMemcpy(enc->d_tables.ptr, tables.data(), tables.size() * 4, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = enc->d_repairs.ensure(m * L);
-    if (err == hipSuccess) err = hipMemset(enc->d_repairs.ptr, 0, m * L);
-    if (err == hipSuccess) err = enc->d_pending.ensure(kPending * L);
-    if (err == hipSuccess) err = enc->h_pending.ensure(kPending * L);
-    if (err == hipSuccess) err = enc->d_ptab.ensure(kPending * 4);
-    if (err == hipSuccess) err = enc->h_repairs.ensure(m * L);
     if (err != hipSuccess) {
         scrs_incr_encoder_destroy(enc);
         if (err == hipErrorOutOfMemory) return st(SCRS_CONFIG_TOO_MANY_SYMBOLS, engine_capacity(p.engine));
@@ -372,6 +364,25 @@ extern "C" scrs_status scrs_incr_encoder_profile(const scrs_incr_encoder *enc, s
     if (!enc) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
     if (!out) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     *out = enc->prof;
+    return ok();
+}
+
+// The feed path's stream, accumulators and staging are made on first use, so
+// an encoder used only through scrs_incr_encode_batch_dev holds its tables
+// alone.
+static scrs_status ensure_feed_state(scrs_incr_encoder *enc) {
+    if (enc->feed_ready) return ok();
+    const uint64_t m = enc->prof.m, L = enc->prof.symbol_len;
+    hipError_t err = hipStreamCreateWithFlags(&enc->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = enc->d_repairs.ensure(m * L);
+    if (err == hipSuccess) err = hipMemsetAsync(enc->d_repairs.ptr, 0, m * L, enc->stream);
+    if (err == hipSuccess) err = enc->d_pending.ensure(kPending * L);
+    if (err == hipSuccess) err = enc->h_pending.ensure(kPending * L);
+    if (err == hipSuccess) err = enc->d_ptab.ensure(kPending * 4);
+    if (err == hipSuccess) err = enc->h_repairs.ensure(m * L);
+    if (err == hipSuccess) err = hipStreamSynchronize(enc->stream);
+    if (err != hipSuccess) return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+    enc->feed_ready = true;
     return ok();
 }
 
@@ -421,6 +432,7 @@ extern "C" scrs_status scrs_incr_feed(scrs_incr_encoder *enc, uint64_t index, co
     if (s.code != SCRS_OK) return s;
     if (!data) return st(SCRS_ERR_INVALID_ARGUMENT, 3);
     DeviceGuard g(enc->device);
+    if ((s = ensure_feed_state(enc)).code != SCRS_OK) return s;
     const uint64_t L = enc->prof.symbol_len, slot = enc->pending.size();
     std::memcpy(enc->h_pending.as<uint8_t>() + slot * L, data, L);
     hipError_t err = hipMemcpyAsync(enc->d_pending.as<uint8_t>() + slot * L, enc->h_pending.as<uint8_t>() + slot * L,
@@ -440,6 +452,7 @@ extern "C" scrs_status scrs_incr_feed_dev(scrs_incr_encoder *enc, uint64_t index
     if (s.code != SCRS_OK) return s;
     if (!data_dev) return st(SCRS_ERR_INVALID_ARGUMENT, 3);
     DeviceGuard g(enc->device);
+    if ((s = ensure_feed_state(enc)).code != SCRS_OK) return s;
     const uint64_t L = enc->prof.symbol_len, slot = enc->pending.size();
     hipStream_t cs = static_cast<hipStream_t>(stream);
     hipError_t err = hipMemcpyAsync(enc->d_pending.as<uint8_t>() + slot * L, data_dev, L, hipMemcpyDeviceToDevice, cs);
@@ -464,6 +477,8 @@ extern "C" scrs_status scrs_incr_repair(scrs_incr_encoder *enc, uint64_t index, 
     if (index >= m) return st(SCRS_ENCODE_INDEX_OUT_OF_RANGE, index, n);
     if (!ptr) return st(SCRS_ERR_INVALID_ARGUMENT, 3);
     DeviceGuard g(enc->device);
+    scrs_status fs = ensure_feed_state(enc);
+    if (fs.code != SCRS_OK) return fs;
     if (!enc->mirror_valid) {
         scrs_status s = flush(enc);
         if (s.code != SCRS_OK) return s;
@@ -483,7 +498,9 @@ extern "C" scrs_status scrs_incr_repairs_dev(scrs_incr_encoder *enc, uint8_t **r
     if (!enc) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
     if (!repairs_dev) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     DeviceGuard g(enc->device);
-    scrs_status s = flush(enc);
+    scrs_status s = ensure_feed_state(enc);
+    if (s.code != SCRS_OK) return s;
+    s = flush(enc);
     if (s.code != SCRS_OK) return s;
     hipEvent_t ev;
     hipError_t err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
@@ -502,6 +519,8 @@ extern "C" uint64_t scrs_incr_fed_count(const scrs_incr_encoder *enc) { return e
 extern "C" scrs_status scrs_incr_reset(scrs_incr_encoder *enc) {
     if (!enc) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
     DeviceGuard g(enc->device);
+    scrs_status fs = ensure_feed_state(enc);
+    if (fs.code != SCRS_OK) return fs;
     hipError_t err = hipStreamSynchronize(enc->stream);
     if (err == hipSuccess)
         err = hipMemsetAsync(enc->d_repairs.ptr, 0, enc->prof.m * enc->prof.symbol_len, enc->stream);

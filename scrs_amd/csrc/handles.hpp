@@ -5,6 +5,7 @@
 #include <list>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/scrs.h"
@@ -24,7 +25,9 @@ struct scrs_batch_encoder {
     int device = 0;
     uint32_t groups = 0;
     scrs::DevBuf d_tables;       // GF(2^8) product tables of all k terms
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // host-slice calls only, created on first use
+    std::once_flag stream_once;
+    hipError_t stream_err = hipSuccess;
     std::mutex mu;               // guards `own` for scratch-less host-slice calls
     scrs_encode_scratch own{};
     void *afft = nullptr;        // AFFT engine state (afft engine only)
@@ -41,7 +44,7 @@ struct scrs_decode_scratch {
 struct scrs_batch_decoder {
     scrs_profile prof{};
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // host-slice calls only, created on first use
     scrs::DecodeEngine eng;
     scrs_decode_scratch own{};
     void *afft = nullptr;        // AFFT engine state (afft engine only)
@@ -97,4 +100,5 @@ struct scrs_incr_encoder {
     std::vector<uint8_t> fed;
     uint64_t fed_count = 0;
     bool mirror_valid = false;
+    bool feed_ready = false;        // stream and accumulators exist (first feed / repair)
 };

@@ -1,5 +1,6 @@
 // server.cpp - serving loop: admission by payload footprint, continuous
 // batching through the batch engine, or one codec call per job.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -34,6 +35,7 @@ struct Request {
 
 struct InFlight {
     hipEvent_t ev = nullptr;
+    hipEvent_t start = nullptr;  // timing: enqueued before the batch's work
     std::vector<Request *> reqs;
     uint64_t bytes = 0;
     uint64_t ring_end = 0;  // plain mode: pointer-ring position released on completion
@@ -69,6 +71,7 @@ struct scrs_server {
     std::deque<scrs_completion> done;
     std::deque<InFlight> inflight;
     uint64_t inflight_bytes = 0, outstanding = 0;
+    scrs_server_stats counters{};
     std::atomic<bool> stop{false};
     std::thread th;
 
@@ -79,12 +82,19 @@ struct scrs_server {
             return e;
         }
         hipEvent_t e = nullptr;
-        (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        (void)hipEventCreate(&e);
         return e;
     }
 
     void finish(InFlight &f, uint64_t t) {
+        float ms = 0;
+        const bool timed = f.start && hipEventElapsedTime(&ms, f.start, f.ev) == hipSuccess;
+        if (f.start) events.push_back(f.start);
         std::lock_guard<std::mutex> lock(mu);
+        if (timed) {
+            counters.gpu_ns += static_cast<uint64_t>(ms * 1e6);
+            counters.max_gpu_ns = std::max<uint64_t>(counters.max_gpu_ns, static_cast<uint64_t>(ms * 1e6));
+        }
         for (Request *r : f.reqs) {
             scrs_completion c;
             c.tag = r->tag;
@@ -182,6 +192,8 @@ void scrs_server::loop() {
             InFlight f;
             f.reqs = batch;
             f.bytes = bytes;
+            f.start = take_event();
+            (void)hipEventRecord(f.start, stream);
             if (cfg.batched) {
                 jobs.clear();
                 for (Request *r : batch) jobs.push_back(r->job);
@@ -192,6 +204,12 @@ void scrs_server::loop() {
             }
             f.ev = take_event();
             (void)hipEventRecord(f.ev, stream);
+            const uint64_t t1 = scrs_now_ns();
+            std::lock_guard<std::mutex> lock(mu);
+            counters.batches++;
+            counters.jobs += batch.size();
+            counters.host_ns += t1 - t;
+            counters.max_batch_jobs = std::max<uint64_t>(counters.max_batch_jobs, batch.size());
             inflight.push_back(std::move(f));
         }
         if (progressed) continue;
@@ -347,5 +365,13 @@ extern "C" scrs_status scrs_server_replay(scrs_server *s, const scrs_job *jobs, 
         if ((i & 63) == 63) drain(0);
     }
     while (got < count) drain(1);
+    return ok();
+}
+
+extern "C" scrs_status scrs_server_get_stats(scrs_server *s, scrs_server_stats *out) {
+    if (!s) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    if (!out) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = s->counters;
     return ok();
 }

@@ -218,3 +218,31 @@ def test_server_admission_budget(torch_cuda):
     torch.cuda.synchronize()
     w.check()
     srv.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_lone_large_jobs_split_terms(torch_cuda, engine):
+    # A batch far too small to fill the device splits each job's term loop
+    # across units whose partial rows are XORed into cleared destinations.
+    torch = torch_cuda
+    rng = np.random.default_rng(int(engine) + 40)
+    m = 32 if engine != Engine.GOOD_CAUCHY else 30
+    p = Profile.resolve(engine, 200 if engine != Engine.GOOD_CAUCHY else 220, m, 65536)
+    k, L = p.k, p.symbol_len
+    data = torch.empty(k * L, dtype=torch.uint8, device="cuda")
+    fill_random_dev(data, data.numel(), 4242 + int(engine))
+    want_rep = encode_alone(torch, p, data)
+    eng = BatchEngine(tile_bytes=4096, cw_per_unit=1)
+    rep = torch.full((m * L,), 0xAB, dtype=torch.uint8, device="cuda")
+    eng.submit([Job.encode(p, data, rep)])
+    torch.cuda.synchronize()
+    assert torch.equal(rep, want_rep)
+    erased = set(rng.choice(k, size=m, replace=False).tolist())
+    idx = [int(x) for x in rng.permutation([x for x in range(k + m) if x not in erased])[:k]]
+    ptrs = [data.data_ptr() + x * L if x < k else rep.data_ptr() + (x - k) * L for x in idx]
+    out = torch.full((k * L,), 0xAB, dtype=torch.uint8, device="cuda")
+    for _ in range(2):  # second pass reuses the recipe
+        out.fill_(0xAB)
+        eng.submit([Job.decode(p, idx, ptrs, out)])
+        torch.cuda.synchronize()
+        assert torch.equal(out, data)
