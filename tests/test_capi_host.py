@@ -100,3 +100,36 @@ def test_status_names():
     lib = _lib.load()
     assert lib.scrs_status_name(0) == b"Ok"
     assert lib.scrs_status_name(32) == b"DecodeError::DuplicateIndex"
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # sizeof/offsetof from a C compiler reading include/*.h, against the
+    # ctypes mirror the Python host uses (and a Rust #[repr(C)] would copy).
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {
+        "scrs_status": _lib.Status, "scrs_profile": _lib.ProfileStruct,
+        "scrs_push_outcome": _lib.PushOutcomeStruct, "scrs_job": _lib.JobStruct,
+        "scrs_batch_config": _lib.BatchConfigStruct, "scrs_batch_stats": _lib.BatchStatsStruct,
+        "scrs_server_config": _lib.ServerConfigStruct, "scrs_server_stats": _lib.ServerStatsStruct,
+        "scrs_completion": _lib.CompletionStruct,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "scrs.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                                 text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)
