@@ -28,6 +28,7 @@ K, M, L, COUNT = 32, 16, 65536, 4096
 SEED_DATA = 0x5C25_0002
 SEED_ERASE = 0x5C25_E7A5
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
 
 
 def erasure_indices(count: int, k: int, m: int, max_r: int, seed: int) -> np.ndarray:
@@ -300,6 +301,45 @@ def serve_workload(args) -> dict:
     }
 
 
+def host_slice_rates(enc, dec, data, rep, idx, sample: int = 32) -> dict:
+    """The drop-in host-slice calls (encode_into_with / decode_into_with) on
+    ordinary host buffers: every call stages its bytes over PCIe and returns
+    with the result in host memory.  Reported beside the device-resident
+    figures, never as the headline value."""
+    h_data = data[: sample * K * L].cpu().numpy().reshape(sample, K * L)
+    h_rep = rep[: sample * M * L].cpu().numpy().reshape(sample, M * L)
+    got = np.empty(M * L, np.uint8)
+    out = np.empty(K * L, np.uint8)
+    es, ds = enc.scratch(), dec.scratch()
+    for c in range(2):  # warm the staging buffers
+        enc.encode_into_with(h_data[c], got, es)
+    t0 = time.perf_counter()
+    for c in range(sample):
+        enc.encode_into_with(h_data[c], got, es)
+    t_enc = time.perf_counter() - t0
+    if not np.array_equal(got, h_rep[sample - 1]):
+        raise SystemExit("host-slice encode differs from the device batch encode")
+    syms = []
+    for c in range(sample):
+        row = []
+        for t in idx[c]:
+            t = int(t)
+            row.append((t, h_data[c, t * L:(t + 1) * L] if t < K else h_rep[c, (t - K) * L:(t - K + 1) * L]))
+        syms.append(row)
+    dec.decode_into_with(syms[0], out, ds)
+    t0 = time.perf_counter()
+    for c in range(sample):
+        dec.decode_into_with(syms[c], out, ds)
+    t_dec = time.perf_counter() - t0
+    if not np.array_equal(out, h_data[sample - 1]):
+        raise SystemExit("host-slice decode differs from the original data")
+    return {
+        "encode_gbs": round(sample * (K + M) * L / t_enc / 1e9, 2),
+        "decode_gbs": round(sample * 2 * K * L / t_dec / 1e9, 2),
+        "sample": f"{sample} codewords, one encode_into_with / decode_into_with call each, pageable host buffers",
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +347,10 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--count", type=int, default=COUNT, help="codewords per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-bitmatrix", action="store_true",
+                    help="skip the bit-matrix (int8 MFMA) kernel leg of the GF(256) workload")
+    ap.add_argument("--no-host-slice", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) encode_into / decode_into leg")
     ap.add_argument("--workload", choices=["gf256", "tower", "afft", "serve"], default="gf256",
                     help="gf256: the headline GF(256) batch line; tower / afft: the GF(2^16) engines; "
                          "serve: the mixed request stream, plain vs batched")
@@ -382,9 +426,51 @@ def main() -> None:
     ms_per_step = wall * 1e3 / args.steps
     value = (enc_bytes + dec_bytes) / (wall / args.steps) / 1e9
 
+    # The second GF(256) kernel on the same workload: the bit-matrix product
+    # on int8 MFMA, timed the same way and checked against the table path.
+    bitmatrix = None
+    if not args.no_bitmatrix:
+        from scrs_amd import Kernel
+        benc, bdec = batch_encoder(prof), batch_decoder(prof)
+        benc.set_kernel(Kernel.BITMATRIX)
+        bdec.set_kernel(Kernel.BITMATRIX)
+        rep_b = torch.empty_like(rep)
+        out_b = torch.empty_like(out)
+        for _ in range(max(1, args.warmup)):
+            benc.encode_batch_dev(data, rep_b, count, stream)
+            bdec.decode_batch_dev(idx, data, rep_b, out_b, count, stream)
+        evb = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+                torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        for e0, e1, e2 in evb:
+            e0.record(stream)
+            benc.encode_batch_dev(data, rep_b, count, stream)
+            e1.record(stream)
+            bdec.decode_batch_dev(idx, data, rep_b, out_b, count, stream)
+            e2.record(stream)
+        torch.cuda.synchronize()
+        if not torch.equal(rep_b, rep) or not torch.equal(out_b, data):
+            raise SystemExit("bit-matrix kernel output differs from the table kernel")
+        b_enc = sum(a.elapsed_time(b) for a, b, _ in evb) / args.steps
+        b_dec = sum(b.elapsed_time(c) for _, b, c in evb) / args.steps
+        # int8 MFMA work per launch: 2 * (8 rows-bits) * (8 source-bits) per
+        # byte column, with rows padded to 16 per group and sources to 8 per chunk.
+        groups = (M + 15) // 16
+        k_pad = (K + 7) // 8 * 8
+        ops = 2.0 * (8 * 16 * groups) * (8 * k_pad) * L * count
+        bitmatrix = {
+            "encode_ms": round(b_enc, 4), "decode_ms": round(b_dec, 4),
+            "encode_gbs": round(enc_bytes / (b_enc / 1e3) / 1e9, 2),
+            "decode_gbs": round(dec_bytes / (b_dec / 1e3) / 1e9, 2),
+            "encode_int8_tops": round(ops / (b_enc / 1e3) / 1e12, 1),
+            "mfma_frac_of_int8_peak": round(ops / (b_enc / 1e3) / INT8_PEAK_OPS, 4),
+        }
+        del rep_b, out_b
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(idx, rep[: 16 * M * L].cpu().numpy())
+    host = host_slice_rates(enc, dec, data, rep, idx) if not args.no_host_slice else None
 
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
     dec_gbs = dec_bytes / (dec_ms / 1e3) / 1e9
@@ -425,6 +511,8 @@ def main() -> None:
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "bitmatrix_kernel": bitmatrix,
+        "host_slice": host,
     }
     print(json.dumps(line))
 

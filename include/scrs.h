@@ -109,6 +109,25 @@ scrs_status scrs_encode_into_with(const scrs_batch_encoder *enc, const uint8_t *
 scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, const uint8_t *data_dev,
                                   uint8_t *repairs_dev, uint64_t count, void *stream);
 
+/* ---- Multiply-XOR kernel of the GF(256) codecs (no reference counterpart) */
+
+typedef enum scrs_kernel {
+    SCRS_KERNEL_TABLE = 0,    /* split-nibble product tables in LDS (default) */
+    SCRS_KERNEL_BITMATRIX = 1 /* GF(2) bit-matrix product on int8 MFMA */
+} scrs_kernel;
+
+/* Select the multiply-XOR kernel of a GF(256) batch encoder or decoder.  Both
+ * kernels produce the same bytes.  BITMATRIX expresses the code as an
+ * (8m) x (8k) 0/1 matrix product with int32 accumulation and parity
+ * extraction; it requires k <= 64 (else SCRS_ERR_CAPACITY{a = 64}) and
+ * symbol_len % 4 == 0 (else SCRS_ERR_INVALID_ARGUMENT{a = 2}).  Calls whose
+ * buffers are not 4-byte aligned, scrs_decode_batch_ptrs_dev and the
+ * streaming codecs use the table kernel.  Tower and AFFT handles:
+ * UnsupportedMode{engine}.  Not to be called while other work uses the
+ * handle. */
+scrs_status scrs_batch_encoder_set_kernel(scrs_batch_encoder *enc, int32_t kernel);
+scrs_status scrs_batch_decoder_set_kernel(scrs_batch_decoder *dec, int32_t kernel);
+
 /* ---- BatchDecoder (codec.rs:157-175) ------------------------------------ */
 
 /* batch_decoder(&profile) (selector.rs:270-279). */

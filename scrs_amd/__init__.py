@@ -20,6 +20,7 @@ from .codec import (  # noqa: F401
     EncodeError,
     Engine,
     Field,
+    Kernel,
     Profile,
     ScrsError,
     batch_decoder,

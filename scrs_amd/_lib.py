@@ -74,6 +74,8 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_batch_encoder_create": (Status, [C.POINTER(ProfileStruct), HP]),
     "scrs_batch_encoder_destroy": (None, [H]),
     "scrs_batch_encoder_profile": (Status, [H, C.POINTER(ProfileStruct)]),
+    "scrs_batch_encoder_set_kernel": (Status, [H, C.c_int32]),
+    "scrs_batch_decoder_set_kernel": (Status, [H, C.c_int32]),
     "scrs_encode_scratch_create": (Status, [H, HP]),
     "scrs_encode_scratch_destroy": (None, [H]),
     "scrs_encode_into": (Status, [H, vp, C.c_uint64, vp, C.c_uint64]),

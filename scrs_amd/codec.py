@@ -38,6 +38,15 @@ class Engine(IntEnum):
         return Field(_lib.load().scrs_engine_field(int(self)))
 
 
+class Kernel(IntEnum):
+    """Multiply-XOR kernel of the GF(256) batch codecs (no reference
+    counterpart): split-nibble LDS tables, or the GF(2) bit-matrix product on
+    int8 MFMA.  Both give the same bytes."""
+
+    TABLE = 0
+    BITMATRIX = 1
+
+
 # ---------------------------------------------------------------------------
 # Errors
 # ---------------------------------------------------------------------------
@@ -275,6 +284,10 @@ class BatchEncoder(_Coded):
         super().__init__(h.value)
         self.profile = profile
 
+    def set_kernel(self, kernel: Kernel) -> None:
+        """Select the multiply-XOR kernel (GF(256) engines only)."""
+        check(_lib.load().scrs_batch_encoder_set_kernel(self._h, int(kernel)))
+
     def scratch(self) -> EncodeScratch:
         h = C.c_void_p()
         check(_lib.load().scrs_encode_scratch_create(self._h, C.byref(h)))
@@ -337,6 +350,10 @@ class BatchDecoder(_Coded):
         check(_lib.load().scrs_batch_decoder_create(C.byref(profile._struct()), C.byref(h)))
         super().__init__(h.value)
         self.profile = profile
+
+    def set_kernel(self, kernel: Kernel) -> None:
+        """Select the multiply-XOR kernel (GF(256) engines only)."""
+        check(_lib.load().scrs_batch_decoder_set_kernel(self._h, int(kernel)))
 
     def scratch(self) -> DecodeScratch:
         h = C.c_void_p()

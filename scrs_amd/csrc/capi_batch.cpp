@@ -18,6 +18,7 @@ scrs_status construction_error(hipError_t err, int32_t engine) {
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+static bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
 
 // Received indices of one codeword: IndexOutOfRange -> DuplicateIndex, in
 // order (batch/codec.rs:304-317; the blanket BatchDecoder's push order,
@@ -250,8 +251,52 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
     a.groups = enc->groups;
     a.L = L;
     a.count = count;
-    hipError_t err = launch_rows_terms_lds(a, vec16, static_cast<hipStream_t>(stream));
+    hipError_t err;
+    if (enc->kernel == SCRS_KERNEL_BITMATRIX && aligned4(data_dev) && aligned4(repairs_dev)) {
+        a.tables = enc->d_bm_tables.as<uint32_t>();
+        err = launch_rows_bitmatrix(a, static_cast<hipStream_t>(stream));
+    } else {
+        err = launch_rows_terms_lds(a, vec16, static_cast<hipStream_t>(stream));
+    }
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
+}
+
+// Validation shared by both set_kernel calls.
+static scrs_status check_kernel(const scrs_profile &p, int32_t kernel) {
+    if (kernel != SCRS_KERNEL_TABLE && kernel != SCRS_KERNEL_BITMATRIX) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    if (!engine_is_gf256(p.engine)) return st(SCRS_CONFIG_UNSUPPORTED_MODE, p.engine);
+    if (kernel == SCRS_KERNEL_BITMATRIX) {
+        if (p.k > 8 * kBitmatrixMaxChunks) return st(SCRS_ERR_CAPACITY, 8 * kBitmatrixMaxChunks);
+        if (p.symbol_len % 4) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    }
+    return ok();
+}
+
+extern "C" scrs_status scrs_batch_encoder_set_kernel(scrs_batch_encoder *enc, int32_t kernel) {
+    if (!enc) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    scrs_status s = check_kernel(enc->prof, kernel);
+    if (s.code != SCRS_OK) return s;
+    if (kernel == SCRS_KERNEL_BITMATRIX && !enc->d_bm_tables.ptr) {
+        const uint32_t k = static_cast<uint32_t>(enc->prof.k), m = static_cast<uint32_t>(enc->prof.m);
+        const std::vector<uint8_t> c = enc->prof.engine == SCRS_ENGINE_GOOD_CAUCHY ? good_cauchy_matrix(k, m)
+                                                                                   : standard_cauchy_matrix(k, m);
+        std::vector<uint8_t> tab(bitmatrix_table_bytes(k, enc->groups));
+        build_bitmatrix_tables(c.data(), k, m, enc->groups, tab.data());
+        DeviceGuard g(enc->device);
+        hipError_t err = enc->d_bm_tables.ensure(tab.size());
+        if (err == hipSuccess) err = hipMemcpy(enc->d_bm_tables.ptr, tab.data(), tab.size(), hipMemcpyHostToDevice);
+        if (err != hipSuccess) return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+    }
+    enc->kernel = kernel;
+    return ok();
+}
+
+extern "C" scrs_status scrs_batch_decoder_set_kernel(scrs_batch_decoder *dec, int32_t kernel) {
+    if (!dec) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
+    scrs_status s = check_kernel(dec->prof, kernel);
+    if (s.code != SCRS_OK) return s;
+    dec->kernel = kernel;
+    return ok();
 }
 
 // ---------------------------------------------------------------------------
@@ -378,7 +423,8 @@ static scrs_status decode_host(scrs_batch_decoder *dec, const uint64_t *indices,
     scrs_status r = dec->afft ? scrs_afft_decode(dec->afft, idx16.data(), sc.d_ptrs.as<const uint8_t *const>(),
                                                  nullptr, nullptr, sc.d_out.as<uint8_t>(), 1, s)
                               : dec->eng.enqueue(idx16.data(), sc.d_ptrs.as<const uint8_t *const>(), nullptr,
-                                                 nullptr, sc.d_out.as<uint8_t>(), 1, s, L % 16 == 0);
+                                                 nullptr, sc.d_out.as<uint8_t>(), 1, s, L % 16 == 0,
+                                                 dec->kernel == SCRS_KERNEL_BITMATRIX);
     if (r.code != SCRS_OK) return r;
     uint32_t flag = 0;
     if ((err = hipMemcpyAsync(sc.h_out.ptr, sc.d_out.ptr, k * L, hipMemcpyDeviceToHost, s)) != hipSuccess ||
@@ -429,8 +475,10 @@ static scrs_status decode_batch(scrs_batch_decoder *dec, const uint16_t *indices
     // only for the layout form, whose alignment is known here.
     const bool vec16 = !payload_ptrs_dev && L % 16 == 0 && aligned16(out_dev) && aligned16(data_dev) &&
                        aligned16(repairs_dev);
+    const bool bitmatrix = dec->kernel == SCRS_KERNEL_BITMATRIX && !payload_ptrs_dev && aligned4(out_dev) &&
+                           aligned4(data_dev) && aligned4(repairs_dev);
     return dec->eng.enqueue(indices, payload_ptrs_dev, data_dev, repairs_dev, out_dev, count,
-                            static_cast<hipStream_t>(stream), vec16);
+                            static_cast<hipStream_t>(stream), vec16, bitmatrix);
 }
 
 extern "C" scrs_status scrs_decode_batch_dev(scrs_batch_decoder *dec, const uint16_t *indices,

@@ -85,8 +85,10 @@ scrs_status DecodeEngine::init(const scrs_profile &p) {
         if (err == hipErrorOutOfMemory) return st(SCRS_CONFIG_TOO_MANY_SYMBOLS, engine_capacity(p.engine));
         return st(SCRS_ERR_DEVICE, err);
     }
-    // Bound the per-launch workspace: product tables dominate.
-    const uint64_t table_bytes = static_cast<uint64_t>(k) * groups * table_group_bytes(gf16);
+    // Bound the per-launch workspace: product tables dominate (the larger
+    // of the two GF(2^8) kernels' layouts).
+    uint64_t table_bytes = cw_table_bytes(false);
+    if (!gf16 && k <= 8 * kBitmatrixMaxChunks && cw_table_bytes(true) > table_bytes) table_bytes = cw_table_bytes(true);
     const uint64_t budget = 1ull << 30;
     chunk = table_bytes ? budget / table_bytes : 65535;
     if (chunk > 65535) chunk = 65535;
@@ -95,9 +97,11 @@ scrs_status DecodeEngine::init(const scrs_profile &p) {
 
 scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs,
                                   const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out,
-                                  uint64_t count, hipStream_t s, bool vec16) {
+                                  uint64_t count, hipStream_t s, bool vec16, bool bitmatrix) {
     if (count == 0) return ok();
-    if (chunk == 0) return st(SCRS_ERR_CAPACITY, static_cast<uint64_t>(k) * groups * table_group_bytes(gf16));
+    if (bitmatrix && (gf16 || k > 8 * kBitmatrixMaxChunks || L % 4)) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    const uint64_t tb = cw_table_bytes(bitmatrix);
+    if (chunk == 0) return st(SCRS_ERR_CAPACITY, tb);
     const uint64_t want = count < chunk ? count : chunk;
     hipError_t err;
     if (want > cap) {
@@ -107,12 +111,13 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
             (err = d_fused.ensure(want * rr * kk * 2)) != hipSuccess ||
             (err = d_missing.ensure(want * rr * 2)) != hipSuccess || (err = d_rows.ensure(want * 4)) != hipSuccess ||
             (err = d_work.ensure(want * work * 2)) != hipSuccess ||
-            (err = d_tables.ensure(want * kk * groups * table_group_bytes(gf16))) != hipSuccess ||
             (err = d_src.ensure(want * kk * 8)) != hipSuccess || (err = d_dst.ensure(want * rr * 8)) != hipSuccess ||
             (err = d_copy.ensure(want * kk * 8)) != hipSuccess)
             return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
         cap = want;
     }
+    if ((err = d_tables.ensure(cap * tb)) != hipSuccess)
+        return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
     for (uint64_t c0 = 0; c0 < count; c0 += cap) {
         const uint64_t cn = count - c0 < cap ? count - c0 : cap;
         // The pinned index buffer is reused: wait for the previous upload.
@@ -155,6 +160,12 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         pa.groups = groups;
         pa.L = L;
         pa.count = cn;
+        if (bitmatrix) {
+            // Bit-matrix operands from the solve; the prepare step then
+            // writes only the pointer tables.
+            if ((err = launch_bitmatrix_tables(pa, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+            pa.fused = nullptr;
+        }
         if ((err = launch_decode_prepare(pa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
         RowsTermsArgs ra;
         ra.src_ptrs = d_src.as<const uint8_t *const>();
@@ -162,13 +173,14 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         ra.copy_ptrs = d_copy.as<uint8_t *const>();
         ra.rows = d_rows.as<uint32_t>();
         ra.tables = d_tables.as<uint32_t>();
-        ra.table_cw_stride = static_cast<uint64_t>(k) * groups * table_group_bytes(gf16) / 4;
+        ra.table_cw_stride = tb / 4;
         ra.T = k;
         ra.rmax = rmax;
         ra.groups = groups;
         ra.L = L;
         ra.count = cn;
-        if ((err = launch_rows_terms_field(gf16, ra, vec16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);
+        if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     }
     return ok();
 }

@@ -60,14 +60,21 @@ struct DecodeEngine {
     // count x k, received order) are already validated.  Received payloads
     // come from `payload_ptrs` (device array, count x k) or from the
     // back-to-back data / repair layout.  Output rows: out + c*k*L + i*L.
+    // `bitmatrix` runs the GF(2^8) multiply-XOR on the bit-matrix kernel
+    // (payload rows and outputs 4-byte aligned, k <= 64).
     scrs_status enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
-                        const uint8_t *repair_base, uint8_t *out, uint64_t count, hipStream_t s, bool vec16);
+                        const uint8_t *repair_base, uint8_t *out, uint64_t count, hipStream_t s, bool vec16,
+                        bool bitmatrix = false);
     // Decode one codeword with a reconstruction recipe computed earlier (its
     // product tables, missing rows and row count on the device): only the
     // pointer tables and the multiply-XOR run.
     scrs_status enqueue_recipe(const uint16_t *indices, const uint32_t *tables, const uint16_t *missing,
                                const uint32_t *rows, const uint8_t *data_base, const uint8_t *repair_base,
                                uint8_t *out, hipStream_t s, bool vec16);
+    // Table bytes per codeword of the table or bit-matrix kernel.
+    uint64_t cw_table_bytes(bool bitmatrix) const {
+        return bitmatrix ? bitmatrix_table_bytes(k, groups) : static_cast<uint64_t>(k) * groups * table_group_bytes(gf16);
+    }
     // Bytes of one codeword's recipe: tables, missing rows, row count.
     uint64_t recipe_table_bytes() const { return static_cast<uint64_t>(k) * groups * table_group_bytes(gf16); }
     // Non-zero if a solve found an inconsistent submatrix since the last reset.

@@ -25,6 +25,8 @@ struct scrs_batch_encoder {
     int device = 0;
     uint32_t groups = 0;
     scrs::DevBuf d_tables;       // GF(2^8) product tables of all k terms
+    int32_t kernel = SCRS_KERNEL_TABLE;
+    scrs::DevBuf d_bm_tables;    // bit-matrix operands (SCRS_KERNEL_BITMATRIX)
     hipStream_t stream = nullptr;  // host-slice calls only, created on first use
     std::once_flag stream_once;
     hipError_t stream_err = hipSuccess;
@@ -48,6 +50,7 @@ struct scrs_batch_decoder {
     scrs::DecodeEngine eng;
     scrs_decode_scratch own{};
     void *afft = nullptr;        // AFFT engine state (afft engine only)
+    int32_t kernel = SCRS_KERNEL_TABLE;
 };
 
 // One cached reconstruction recipe: the device product tables, row count
