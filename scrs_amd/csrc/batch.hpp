@@ -20,6 +20,7 @@ enum BatchRangeFlags : uint32_t {
     kRangeVec16 = 1,   // every row address and L are multiples of 16
     kRangeDecode = 2,  // copies of present data rows (copy pointers)
     kRangeGf16 = 4,    // GF(2^16) tables and elements
+    kRangeDst4 = 8,    // destination rows and L are multiples of 4 (term split allowed)
 };
 
 struct BatchRange {

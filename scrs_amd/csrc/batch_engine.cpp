@@ -24,9 +24,11 @@ constexpr uint32_t kSolveGrid = 256;
 constexpr uint16_t kDataSlot = 0xFFFF;
 constexpr uint64_t kTuneMinJobs = 32;
 constexpr uint64_t kFillUnits = 512;  // two units per CU
+constexpr uint32_t kMinSegTerms = 8;   // shortest term segment of a split range
 
 uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t{15}; }
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
 uint32_t log2_floor(uint64_t x) {
     uint32_t d = 0;
     while (x > 1) {
@@ -261,14 +263,18 @@ CacheEntry *BatchEngine::encoder_tables(const scrs_profile &p, Plan &plan, scrs_
 
 scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p) {
     p = Plan();
-    // Encode jobs of one geometry, length and alignment share a range.
-    std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t, bool>, std::vector<uint64_t>> groups;
+    // Encode jobs of one geometry, length and alignment class share a range
+    // (class 2: rows 16-byte aligned; 1: destination rows 4-byte aligned).
+    std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t, int>, std::vector<uint64_t>> groups;
     std::vector<uint64_t> decodes;
     for (uint64_t i = 0; i < count; i++) {
         const scrs_job &j = jobs[i];
         if (j.op == SCRS_JOB_ENCODE) {
-            const bool v = j.profile.symbol_len % 16 == 0 && aligned16(j.src) && aligned16(j.dst);
-            groups[std::make_tuple(j.profile.engine, j.profile.k, j.profile.m, j.profile.symbol_len, v)].push_back(i);
+            const uint64_t L = j.profile.symbol_len;
+            const int v = (L % 16 == 0 && aligned16(j.src) && aligned16(j.dst)) ? 2
+                          : (L % 4 == 0 && aligned4(j.dst))                     ? 1
+                                                                                : 0;
+            groups[std::make_tuple(j.profile.engine, j.profile.k, j.profile.m, L, v)].push_back(i);
         } else {
             decodes.push_back(i);
         }
@@ -299,7 +305,9 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
         r.tiles = static_cast<uint32_t>((L + tile - 1) / tile);
         r.cw_count = static_cast<uint32_t>(kv.second.size());
         r.ncw = std::min<uint32_t>(cw, r.cw_count);
-        r.flags = (std::get<4>(kv.first) ? kRangeVec16 : 0u) | (engine_is_gf256(pr.engine) ? 0u : kRangeGf16);
+        const int cls = std::get<4>(kv.first);
+        r.flags = (cls == 2 ? kRangeVec16 | kRangeDst4 : cls == 1 ? kRangeDst4 : 0u) |
+                  (engine_is_gf256(pr.engine) ? 0u : kRangeGf16);
         Plan::RangeRef ref;
         ref.src = p.ptrs.size();
         for (uint64_t ji : kv.second)
@@ -339,7 +347,8 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
         br.tiles = static_cast<uint32_t>((L + tile - 1) / tile);
         br.cw_count = 1;
         br.ncw = 1;
-        br.flags = kRangeDecode | (vec ? kRangeVec16 : 0u) | (gf16 ? kRangeGf16 : 0u);
+        const bool dst4 = L % 4 == 0 && aligned4(j.dst);
+        br.flags = kRangeDecode | (vec ? kRangeVec16 : 0u) | (dst4 ? kRangeDst4 : 0u) | (gf16 ? kRangeGf16 : 0u);
         if (r) {
             CacheEntry *e;
             auto it = recipe_cache_.find(key);
@@ -397,21 +406,22 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
 }
 
 // A batch too small to give every CU work splits the term loop of its
-// larger aligned ranges: each segment's unit XORs its partial rows into
-// destinations the table launch clears first.  Latency of a lone large
-// codeword drops by the split factor; a full batch is left as it is.
+// ranges with dword-aligned destination rows: each segment's unit XORs its
+// partial rows into destinations the solve launch clears first.  A unit's
+// term loop waits out one memory latency per term, so the latency of a lone
+// codeword drops with the segment length (kMinSegTerms at the shortest); a
+// full batch is left as it is.
 void BatchEngine::split_terms(Plan &p, uint64_t &units) {
     if (units >= kFillUnits || units == 0) return;
     const uint64_t factor = (kFillUnits + units - 1) / units;
     bool any = false;
     for (size_t i = 0; i < p.ranges.size(); i++) {
         BatchRange &r = p.ranges[i];
-        const uint32_t chunk = (r.flags & kRangeGf16) ? 16u : 32u;
-        if (!(r.flags & kRangeVec16) || r.rows == 0 || r.T <= chunk) continue;
-        const uint32_t max_segs = (r.T + chunk - 1) / chunk;
+        if (!(r.flags & kRangeDst4) || r.rows == 0 || r.T < 2 * kMinSegTerms) continue;
+        const uint32_t max_segs = r.T / kMinSegTerms;
         const uint32_t want = static_cast<uint32_t>(std::min<uint64_t>(max_segs, factor));
         if (want < 2) continue;
-        r.seg_terms = ((r.T + want - 1) / want + chunk - 1) / chunk * chunk;
+        r.seg_terms = (r.T + want - 1) / want;
         r.segs = (r.T + r.seg_terms - 1) / r.seg_terms;
         if (r.segs < 2) {
             r.segs = 1;

@@ -150,9 +150,14 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
     }
     // Destinations of term-split ranges start from zero.
     for (uint32_t z = blockIdx.x; z < hdr->nzero; z += gridDim.x) {
-        uint4 *dst = reinterpret_cast<uint4 *>(hdr->zero[2 * z]);
-        const uint64_t n16 = hdr->zero[2 * z + 1] / 16;
-        for (uint64_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = make_uint4(0, 0, 0, 0);
+        const uint64_t addr = hdr->zero[2 * z], len = hdr->zero[2 * z + 1];
+        if (((addr | len) & 15) == 0) {
+            uint4 *dst = reinterpret_cast<uint4 *>(addr);
+            for (uint64_t i = threadIdx.x; i < len / 16; i += blockDim.x) dst[i] = make_uint4(0, 0, 0, 0);
+        } else {  // dword-aligned rows, the only other kind a split range has
+            uint32_t *dst = reinterpret_cast<uint32_t *>(addr);
+            for (uint64_t i = threadIdx.x; i < len / 4; i += blockDim.x) dst[i] = 0;
+        }
     }
 }
 
@@ -162,13 +167,6 @@ __global__ void batch_reset_kernel(uint32_t *counter) {
 
 // ---- multiply-XOR ----------------------------------------------------------
 
-// XOR a 16-byte row slice into device memory (term-split partial sums).
-__device__ __forceinline__ void xor16(uint8_t *dst, uint64_t p, const uint32_t (&o)[4]) {
-    uint32_t *d = reinterpret_cast<uint32_t *>(dst + p);
-#pragma unroll
-    for (int j = 0; j < 4; j++) atomicXor(d + j, o[j]);
-}
-
 template <bool VEC, bool DEC, int RW, int TPU>
 __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here, uint64_t p,
                       uint32_t tb, uint32_t te, uint8_t *lds) {
@@ -176,7 +174,7 @@ __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t 
     const bool active = p < R.L;
     const bool single = te - tb <= kChunk8;
     const bool do_copy = DEC && rg == 0;
-    const bool split = VEC && R.segs > 1;
+    const bool split = R.segs > 1;  // rows are dword-aligned (planner)
     for (uint32_t c = cw0; c < cw0 + ncw; c++) {
         uint32_t acc[16][D];
 #pragma unroll
@@ -218,7 +216,7 @@ __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t 
             for (int jj = 0; jj < 4; jj++)
                 o[jj] = ((acc[4 * jj + 0][d] >> s) & 0xffu) | (((acc[4 * jj + 1][d] >> s) & 0xffu) << 8) |
                         (((acc[4 * jj + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * jj + 3][d] >> s) & 0xffu) << 24);
-            if (split) xor16(dst, p, o);
+            if (split) xor16_atomic(dst, p, R.L, o);
             else store16<VEC>(dst, p, R.L, o);
         }
     }
@@ -232,7 +230,7 @@ __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t
     const bool active = p < R.L;
     const bool single = te - tb <= kChunk16;
     const bool do_copy = DEC && rg == 0;
-    const bool split = VEC && R.segs > 1;
+    const bool split = R.segs > 1;  // rows are dword-aligned (planner)
     for (uint32_t c = cw0; c < cw0 + ncw; c++) {
         uint32_t acc[8][D];
 #pragma unroll
@@ -275,7 +273,7 @@ __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t
 #pragma unroll
             for (int jj = 0; jj < 4; jj++)
                 o[jj] = ((acc[2 * jj][d] >> sh) & 0xffffu) | (((acc[2 * jj + 1][d] >> sh) & 0xffffu) << 16);
-            if (split) xor16(dst, p, o);
+            if (split) xor16_atomic(dst, p, R.L, o);
             else store16<VEC>(dst, p, R.L, o);
         }
     }

@@ -10,38 +10,70 @@
 namespace scrs {
 namespace rowsdev {
 
+// 16 bytes at column p of a row (p < L, p a multiple of 16).  VEC: the row
+// address and L are multiples of 16, known at compile time.  Otherwise the
+// width follows the row's alignment at run time (row pointers are
+// wave-uniform, so the branch is too): one 16-byte load when the address and
+// L are multiples of 16; else the five aligned dwords covering the 16 bytes,
+// funnel-shifted into place.  An aligned dword holding at least one byte of
+// the row lies in the same page as that byte, so the loads never leave
+// mapped memory; bytes past L are don't-cares (never stored).
 template <bool VEC>
 __device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t L, uint32_t (&w)[4]) {
-    if (VEC) {
+    if (VEC || (((reinterpret_cast<uint64_t>(src) | L) & 15) == 0)) {
         const uint4 v = *reinterpret_cast<const uint4 *>(src + p);
         w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
     } else {
+        const uint8_t *at = src + p;
+        const uint32_t sh = static_cast<uint32_t>(reinterpret_cast<uint64_t>(at) & 3u);
+        const uint32_t *al = reinterpret_cast<const uint32_t *>(at - sh);
+        const uint8_t *end = src + L;
+        uint32_t d[5];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uint32_t d = 0;
+        for (int i = 0; i < 5; i++) d[i] = reinterpret_cast<const uint8_t *>(al + i) < end ? al[i] : 0u;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const uint64_t q = p + j * 4 + b;
-                if (q < L) d |= static_cast<uint32_t>(src[q]) << (8 * b);
-            }
-            w[j] = d;
-        }
+        for (int j = 0; j < 4; j++) w[j] = __builtin_amdgcn_alignbyte(d[j + 1], d[j], sh);
     }
 }
 
+// Byte stores of 16 bytes at column p, none at or past L.  Out of line: the
+// unaligned-row case is rare and its code would otherwise inflate the
+// register budget of every kernel that inlines store16.
+__device__ __noinline__ void store16_bytes(uint8_t *dst, uint64_t p, uint64_t L, uint32_t w0, uint32_t w1,
+                                            uint32_t w2, uint32_t w3) {
+    const uint32_t w[4] = {w0, w1, w2, w3};
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint64_t q = p + j * 4 + b;
+            if (q < L) dst[q] = static_cast<uint8_t>(w[j] >> (8 * b));
+        }
+}
+
+// Store counterpart of load16: 16-byte or dword stores where the row's
+// alignment allows, bytes otherwise; nothing at or past L is written.
 template <bool VEC>
 __device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
-    if (VEC) {
+    const uint64_t al = reinterpret_cast<uint64_t>(dst) | L;
+    if (VEC || (al & 15) == 0) {
         *reinterpret_cast<uint4 *>(dst + p) = make_uint4(w[0], w[1], w[2], w[3]);
-    } else {
+    } else if ((al & 3) == 0) {
 #pragma unroll
         for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const uint64_t q = p + j * 4 + b;
-                if (q < L) dst[q] = static_cast<uint8_t>(w[j] >> (8 * b));
-            }
+            if (p + 4 * j < L) *reinterpret_cast<uint32_t *>(dst + p + 4 * j) = w[j];
+    } else {
+        store16_bytes(dst, p, L, w[0], w[1], w[2], w[3]);
     }
+}
+
+// XOR 16 bytes into a row whose address and L are multiples of 4 (term-split
+// partial sums): one atomic per dword inside the row.
+__device__ __forceinline__ void xor16_atomic(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst + p);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (p + 4 * j < L) atomicXor(d + j, w[j]);
 }
 
 // Entry of RW bytes at byte offset `off` of the LDS chunk.
