@@ -46,7 +46,11 @@ std::vector<std::pair<uint32_t, uint32_t>> plan_bits(uint32_t D, uint32_t cap) {
 
 // Largest pass that still fits every table a single-pass decode stages.
 constexpr uint32_t kSinglePassLog = 7;
-constexpr uint32_t kPassLog = 8;
+#ifndef SCRS_AFFT_PASS_LOG
+#define SCRS_AFFT_PASS_LOG 8
+#endif
+// Levels per butterfly pass at most.
+constexpr uint32_t kPassLog = SCRS_AFFT_PASS_LOG;
 constexpr uint64_t kWorkBudget = 1ull << 30;
 
 }  // namespace

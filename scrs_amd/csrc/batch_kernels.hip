@@ -245,8 +245,9 @@ __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t
                 for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += TPU) {
                     const uint32_t t = i / (64 * D), rem = i % (64 * D);
                     const uint32_t e = rem / D, d = rem % D;
-                    reinterpret_cast<uint32_t *>(lds)[i] = *reinterpret_cast<const uint32_t *>(
-                        tb + (static_cast<uint64_t>(t0 + t) * R.groups + rg) * 2048 + e * 32 + d * 4);
+                    reinterpret_cast<uint32_t *>(lds)[t * 64 * D + lds16_index<EB>(e, d)] =
+                        *reinterpret_cast<const uint32_t *>(
+                            tb + (static_cast<uint64_t>(t0 + t) * R.groups + rg) * 2048 + e * 32 + d * 4);
                 }
                 __syncthreads();
             }

@@ -47,7 +47,7 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
             const uint32_t t = i / (64 * D), rem = i % (64 * D);
             const uint32_t e = rem / D, d = rem % D;
             const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
-            reinterpret_cast<uint32_t *>(lds)[i] = *reinterpret_cast<const uint32_t *>(
+            reinterpret_cast<uint32_t *>(lds)[t * 64 * D + lds16_index<EB>(e, d)] = *reinterpret_cast<const uint32_t *>(
                 tab_cw + (static_cast<uint64_t>(tt) * a.groups + rg) * kGroupBytes + e * 32 + d * 4);
         }
         __syncthreads();

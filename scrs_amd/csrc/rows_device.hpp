@@ -145,11 +145,23 @@ __device__ __forceinline__ uint32_t load_uniform(const uint32_t *ptr) { return *
 
 // ---- GF(2^16): entries of EB = 2 * rows bytes, 64 entries per term ----
 
+// LDS dword index of dword d of entry e = 16q + n (nibble position q, value
+// n) of one term's staged table.  32-byte entries are split into two 16-byte
+// halves kept in separate 256-byte runs per nibble position: a lane's
+// 16-byte read then always falls in one 256-byte run of 16 entries, which
+// spans the LDS banks once, so data-dependent lookups never conflict.
+// Smaller entries keep 16 per run already.
+template <int EB>
+__device__ __forceinline__ uint32_t lds16_index(uint32_t e, uint32_t d) {
+    if constexpr (EB == 32) return (e >> 4) * 128u + (d >> 2) * 64u + (e & 15u) * 4u + (d & 3u);
+    else return e * (EB / 4) + d;
+}
+
 template <int EB>
 __device__ __forceinline__ void load_entry(const uint8_t *lds, uint32_t off, uint32_t (&d)[EB / 4]) {
     if constexpr (EB == 32) {
         const uint4 a = *reinterpret_cast<const uint4 *>(lds + off);
-        const uint4 b = *reinterpret_cast<const uint4 *>(lds + off + 16);
+        const uint4 b = *reinterpret_cast<const uint4 *>(lds + off + 256);
         d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
         d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
     } else if constexpr (EB == 16) {
@@ -162,10 +174,13 @@ __device__ __forceinline__ void load_entry(const uint8_t *lds, uint32_t off, uin
 }
 
 // Fold the 8 elements of a 16-byte slice.  `tab` = the term's 64 entries
-// (nibble position q major, value n minor) of EB bytes.
+// (nibble position q major, value n minor) of EB bytes, laid out by
+// lds16_index: nibble position q starts at byte 16 * EB * q, value n at 16n
+// for 32-byte entries (halves 256 bytes apart), EB * n otherwise.
 template <int EB>
 __device__ __forceinline__ void fold_term16(uint32_t (&acc)[8][EB / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
-    constexpr int SH = EB == 32 ? 5 : (EB == 16 ? 4 : 3);
+    constexpr int SH = EB == 32 ? 4 : (EB == 16 ? 4 : 3);  // log2 of the stride between values n
+    constexpr uint32_t Q = 16u * EB;                         // bytes per nibble position
 #pragma unroll
     for (int j = 0; j < 4; j++) {
 #pragma unroll
@@ -173,9 +188,9 @@ __device__ __forceinline__ void fold_term16(uint32_t (&acc)[8][EB / 4], const ui
             const uint32_t x = (w[j] >> (16 * h)) & 0xffffu;
             uint32_t e0[EB / 4], e1[EB / 4], e2[EB / 4], e3[EB / 4];
             load_entry<EB>(tab, ((x & 15u) << SH), e0);
-            load_entry<EB>(tab, (16u << SH) + (((x >> 4) & 15u) << SH), e1);
-            load_entry<EB>(tab, (32u << SH) + (((x >> 8) & 15u) << SH), e2);
-            load_entry<EB>(tab, (48u << SH) + ((x >> 12) << SH), e3);
+            load_entry<EB>(tab, Q + (((x >> 4) & 15u) << SH), e1);
+            load_entry<EB>(tab, 2 * Q + (((x >> 8) & 15u) << SH), e2);
+            load_entry<EB>(tab, 3 * Q + ((x >> 12) << SH), e3);
             uint32_t(&a)[EB / 4] = acc[2 * j + h];
 #pragma unroll
             for (int d = 0; d < EB / 4; d++) a[d] = xor3(xor3(a[d], e0[d], e1[d]), e2[d], e3[d]);
