@@ -225,8 +225,11 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a) {
         for (int j = 0; j < R; j++) {
             const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
             uint32_t acc = 0;
-            for (uint32_t beta = 0; beta < g; beta++)
-                if (!((p >> beta) & 1u))
+            // Unrolled over the at most 8 pass bits so the coefficient reads
+            // and table lookups of all terms are in flight together.
+#pragma unroll
+            for (uint32_t beta = 0; beta < 8; beta++)
+                if (beta < g && !((p >> beta) & 1u))
                     acc ^= gmul2(der_t + (b_lo + beta) * kTab, data[(p | (1u << beta)) * kTileDwords + lane]);
             if (a.aux_base && valid) {
                 const uint32_t r = base | (p << b_lo);
