@@ -59,6 +59,14 @@ uint32_t scrs_abi_version(void);
 /* Human-readable name of a status code (no reference counterpart). */
 const char *scrs_status_name(int32_t code);
 
+/* Device and pinned-host allocations the library has made since it was
+ * loaded (no reference counterpart).  The observable of the zero-allocation
+ * steady state the reference tests with a counting allocator
+ * (tests/zero_alloc.rs:48-146): once a codec and its scratch have run one
+ * block, further blocks of the same geometry and receipt pattern leave it
+ * unchanged. */
+uint64_t scrs_device_alloc_count(void);
+
 /* ---- Profile selection ------------------------------------------------ */
 
 /* Engine::field (codec.rs:42-50).  Returns -1 for an unknown engine. */

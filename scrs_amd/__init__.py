@@ -29,5 +29,6 @@ from .codec import (  # noqa: F401
     incremental_encoder,
     fill_random_dev,
     now_ns,
+    device_alloc_count,
     recommended_gf16_engine,
 )

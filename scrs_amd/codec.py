@@ -712,6 +712,11 @@ class Server(_Handle):
         return res
 
 
+def device_alloc_count() -> int:
+    """Device and pinned-host allocations the library has made so far."""
+    return int(_lib.load().scrs_device_alloc_count())
+
+
 def now_ns() -> int:
     return int(_lib.load().scrs_now_ns())
 

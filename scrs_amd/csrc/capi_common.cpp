@@ -2,6 +2,8 @@
 #include "handles.hpp"
 #include "status.hpp"
 
+extern "C" uint64_t scrs_device_alloc_count(void) { return scrs::alloc_counter().load(std::memory_order_relaxed); }
+
 extern "C" const char *scrs_status_name(int32_t code) {
     switch (code) {
         case SCRS_OK: return "Ok";

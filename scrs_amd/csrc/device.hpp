@@ -3,11 +3,19 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <utility>
 
 namespace scrs {
+
+// Device and pinned-host allocations made by the library so far (the
+// observable of the zero-allocation steady state).
+inline std::atomic<uint64_t> &alloc_counter() {
+    static std::atomic<uint64_t> n{0};
+    return n;
+}
 
 // Device allocation that only grows.
 struct DevBuf {
@@ -25,6 +33,7 @@ struct DevBuf {
     hipError_t ensure(size_t n) {
         if (n <= size) return hipSuccess;
         release();
+        alloc_counter().fetch_add(1, std::memory_order_relaxed);
         hipError_t err = hipMalloc(&ptr, n);
         if (err != hipSuccess) {
             ptr = nullptr;
@@ -52,6 +61,7 @@ struct HostBuf {
     hipError_t ensure(size_t n) {
         if (n <= size) return hipSuccess;
         release();
+        alloc_counter().fetch_add(1, std::memory_order_relaxed);
         hipError_t err = hipHostMalloc(&ptr, n, hipHostMallocDefault);
         if (err != hipSuccess) {
             ptr = nullptr;
