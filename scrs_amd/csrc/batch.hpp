@@ -68,7 +68,8 @@ constexpr uint32_t kBatchTileBytes[3] = {1024, 2048, 4096};
 hipError_t launch_batch_reset(uint32_t *counter, hipStream_t s);
 // The launches read every size and array address from the device header, so
 // their arguments stay fixed across batches.
-hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, uint32_t grid, hipStream_t s);
+hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, const uint16_t *gf16_log_exp,
+                              uint32_t grid, hipStream_t s);
 hipError_t launch_batch_rows(const BatchHeader *hdr, uint32_t *counter, uint32_t tile_bytes, uint32_t grid,
                              hipStream_t s);
 // Persistent grid that fills the device for the given tile width.

@@ -101,6 +101,7 @@ scrs_status BatchEngine::init(const scrs_batch_config &cfg) {
     gf8_exp_log(t, t + 512);
     if (err == hipSuccess && (err = gf8_.ensure(768)) == hipSuccess)
         err = hipMemcpy(gf8_.ptr, t, 768, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = gf16_log_exp_device(&gf16_log_exp_);
     if (err == hipSuccess) err = arena_.init(cfg_.cache_bytes);
     if (err == hipSuccess) err = hipEventCreateWithFlags(&last_done_, hipEventDisableTiming);
     for (Slot &s : slots_) {
@@ -457,7 +458,7 @@ hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
         if (err != hipSuccess) return err;
         // The queue counter is cleared by a kernel node: no memset nodes.
         hipError_t e1 = launch_batch_reset(slot.counter.as<uint32_t>(), capture_);
-        hipError_t e2 = launch_batch_solve(hdr, gf8_.as<uint8_t>(), kSolveGrid, capture_);
+        hipError_t e2 = launch_batch_solve(hdr, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, capture_);
         hipError_t e3 = launch_batch_rows(hdr, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), capture_);
         err = hipStreamEndCapture(capture_, &graph);
         if (err == hipSuccess) err = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3);
@@ -536,7 +537,7 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     } else {
         const BatchHeader *dh = slot.dev.as<BatchHeader>();
         if ((err = launch_batch_reset(slot.counter.as<uint32_t>(), s)) == hipSuccess &&
-            (err = launch_batch_solve(dh, gf8_.as<uint8_t>(), kSolveGrid, s)) == hipSuccess)
+            (err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, s)) == hipSuccess)
             err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), s);
     }
     if (err != hipSuccess) return err;

@@ -90,6 +90,7 @@ class BatchEngine {
     hipStream_t capture_ = nullptr;
     DevBuf coords_[3];   // Standard, Good, Tower coordinates of every wire index
     DevBuf gf8_;         // exp (512) + log (256)
+    const uint16_t *gf16_log_exp_ = nullptr;  // GF(2^16) log / exp (shared per device)
     Slot slots_[2];
     uint32_t next_slot_ = 0;
     uint64_t seq_ = 0;

@@ -26,8 +26,13 @@ constexpr uint32_t kLds = 32768;
 // Closed-form Cauchy solve of one decode job and the expansion of its fused
 // coefficients into product tables; the same factorisation as the
 // single-geometry solve (decoder/cauchy_inverse.rs:29-170).
-template <typename F>
-__device__ void solve_job(const BatchSolve &j, const F &f) {
+// In the log domain (rf, cf and q hold discrete logs): long products become
+// sums of independent lookups.  Coordinates are distinct, so no difference
+// is zero.
+template <typename LF>
+__device__ void solve_job(const BatchSolve &j, const LF &L) {
+    constexpr uint32_t O = LF::kOrder;
+    auto quot = [](uint32_t num, uint32_t den) { return (num % O + O - den % O) % O; };
     const uint32_t k = j.k, r = j.r, tid = threadIdx.x, nt = blockDim.x;
     uint32_t *rv = j.work, *cv = rv + r, *rf = cv + r, *cf = rf + r, *q = cf + r;
     for (uint32_t t = tid; t < k; t += nt) {
@@ -37,35 +42,41 @@ __device__ void solve_job(const BatchSolve &j, const F &f) {
     for (uint32_t i = tid; i < r; i += nt) cv[i] = j.coords[j.missing[i]];
     __syncthreads();
     for (uint32_t i = tid; i < r; i += nt) {
-        uint32_t cross = 1, within = 1;
+        uint32_t cross = 0, within = 0;
         const uint32_t x = rv[i];
-        for (uint32_t h = 0; h < r; h++) cross = f.mul(cross, x ^ cv[h]);
+        for (uint32_t h = 0; h < r; h++) cross += L.lg(x ^ cv[h]);
         for (uint32_t l = 0; l < r; l++)
-            if (l != i) within = f.mul(within, x ^ rv[l]);
-        rf[i] = f.mul(cross, f.inv(within));
-        uint32_t c2 = 1, w2 = 1;
+            if (l != i) within += L.lg(x ^ rv[l]);
+        rf[i] = quot(cross, within);
+        uint32_t c2 = 0, w2 = 0;
         const uint32_t y = cv[i];
-        for (uint32_t l = 0; l < r; l++) c2 = f.mul(c2, y ^ rv[l]);
+        for (uint32_t l = 0; l < r; l++) c2 += L.lg(y ^ rv[l]);
         for (uint32_t h = 0; h < r; h++)
-            if (h != i) w2 = f.mul(w2, y ^ cv[h]);
-        cf[i] = f.mul(c2, f.inv(w2));
+            if (h != i) w2 += L.lg(y ^ cv[h]);
+        cf[i] = quot(c2, w2);
     }
     for (uint32_t t = tid; t < k; t += nt) {
         if (j.slot[t] != 0xFFFFu) continue;
         const uint32_t z = j.coords[j.idx[t]];
-        uint32_t num = 1, den = 1;
-        for (uint32_t h = 0; h < r; h++) num = f.mul(num, z ^ cv[h]);
-        for (uint32_t l = 0; l < r; l++) den = f.mul(den, z ^ rv[l]);
-        q[t] = f.mul(num, f.inv(den));
+        uint32_t num = 0, den = 0;
+        for (uint32_t h = 0; h < r; h++) num += L.lg(z ^ cv[h]);
+        for (uint32_t l = 0; l < r; l++) den += L.lg(z ^ rv[l]);
+        q[t] = quot(num, den);
     }
     __syncthreads();
     for (uint32_t x = tid; x < r * k; x += nt) {
         const uint32_t jj = x / k, t = x % k;
         const uint32_t s = j.slot[t];
-        uint32_t v;
-        if (s != 0xFFFFu) v = f.mul(f.mul(rf[s], cf[jj]), f.inv(cv[jj] ^ rv[s]));
-        else v = f.mul(f.mul(cf[jj], q[t]), f.inv(j.coords[j.idx[t]] ^ cv[jj]));
-        j.fused[jj * k + t] = static_cast<uint16_t>(v);
+        uint32_t e, d;
+        if (s != 0xFFFFu) {
+            e = rf[s] + cf[jj];
+            d = L.lg(cv[jj] ^ rv[s]);
+        } else {
+            e = cf[jj] + q[t];
+            d = L.lg(j.coords[j.idx[t]] ^ cv[jj]);
+        }
+        e = e >= d ? e - d : e + O - d;  // < 2 * O
+        j.fused[jj * k + t] = static_cast<uint16_t>(L.ex(e));
     }
     __syncthreads();
 }
@@ -126,7 +137,8 @@ __device__ void tables_job16(const BatchSolve &j, const FieldGf65536 &f) {
     }
 }
 
-__global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr, const uint8_t *gf8) {
+__global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr, const uint8_t *gf8,
+                                                         const uint16_t *gf16_log_exp) {
     __shared__ uint8_t exp[512];
     __shared__ uint8_t log[256];
     for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) exp[i] = gf8[i];
@@ -138,12 +150,10 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
     for (uint32_t jx = blockIdx.x; jx < n; jx += gridDim.x) {
         const BatchSolve j = solves[jx];
         if (j.gf16) {
-            const FieldGf65536 f{t8};
-            if (!j.encode) solve_job(j, f);
-            tables_job16(j, f);
+            if (!j.encode) solve_job(j, LogGf65536{gf16_log_exp});
+            tables_job16(j, FieldGf65536{t8});
         } else {
-            const FieldGf256 f{t8};
-            if (!j.encode) solve_job(j, f);
+            if (!j.encode) solve_job(j, LogGf256{t8});
             tables_job8(j, t8);
         }
         __syncthreads();
@@ -357,8 +367,9 @@ hipError_t launch_batch_reset(uint32_t *counter, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables);
+hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, const uint16_t *gf16_log_exp,
+                              uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables, gf16_log_exp);
     return hipGetLastError();
 }
 

@@ -47,7 +47,34 @@ struct FieldGf65536 {
     }
 };
 
+// Log-domain access for the closed-form solve, where every quantity is a
+// product or quotient of coordinate differences: logs add, so long products
+// become sums of independent lookups instead of chains of dependent
+// multiplications.  ex() takes exponents below 2 * kOrder.
+struct LogGf256 {
+    Gf8Tables t;  // exp doubled (512), log (256) -- in LDS
+    static constexpr uint32_t kOrder = 255;
+    __device__ __forceinline__ uint32_t lg(uint32_t x) const { return t.log[x]; }
+    __device__ __forceinline__ uint32_t ex(uint32_t e) const { return t.exp[e]; }
+};
+
+// GF(2^16) log / exp over the generator g = 0x0108 (order 65535,
+// algebra/gf65536.rs:26): log[65536] then exp[2 * 65535], in device memory.
+struct LogGf65536 {
+    const uint16_t *tab;
+    static constexpr uint32_t kOrder = 65535;
+    __device__ __forceinline__ uint32_t lg(uint32_t x) const { return tab[x]; }
+    __device__ __forceinline__ uint32_t ex(uint32_t e) const { return tab[65536u + e]; }
+};
+
+// Entries of the GF(2^16) log / exp table (log 65536, exp 2 * 65535).
+constexpr uint32_t kGf16LogExpEntries = 65536u + 2u * 65535u;
+
 // Host copy of the GF(2^8) exp (512, doubled) / log (256) tables.
 void gf8_exp_log(uint8_t exp[512], uint8_t log[256]);
+
+// Device copy of the GF(2^16) log / exp table for the current device, built
+// on first use and kept until the process ends.
+hipError_t gf16_log_exp_device(const uint16_t **out);
 
 }  // namespace scrs

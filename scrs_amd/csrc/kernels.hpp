@@ -59,6 +59,7 @@ struct CauchySolveArgs {
     uint32_t *error_flag = nullptr;
     uint32_t k = 0, m = 0, rmax = 0;
     uint64_t count = 0;
+    const uint16_t *gf16_log_exp = nullptr;  // filled by the launcher (GF(2^16))
 };
 uint64_t cauchy_solve_work_elems(uint32_t k, uint32_t rmax);
 hipError_t launch_cauchy_solve(const CauchySolveArgs &a, bool gf16, hipStream_t s);

@@ -16,6 +16,12 @@
 // fused coefficients are then expanded into split-nibble product tables.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "device.hpp"
+#include "field.hpp"
 #include "gf_device.hpp"
 #include "kernels.hpp"
 
@@ -50,14 +56,29 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t flag, uint32_t *buf, uin
     return incl - flag;
 }
 
-template <typename F>
+template <bool GF16> struct SolveLog;
+template <> struct SolveLog<false> {
+    using type = LogGf256;
+    __device__ static type make(const uint8_t *exp, const uint8_t *log, const CauchySolveArgs &) {
+        return LogGf256{{exp, log}};
+    }
+};
+template <> struct SolveLog<true> {
+    using type = LogGf65536;
+    __device__ static type make(const uint8_t *, const uint8_t *, const CauchySolveArgs &a) {
+        return LogGf65536{a.gf16_log_exp};
+    }
+};
+
+template <bool GF16>
 __global__ __launch_bounds__(kSolveThreads) void cauchy_solve_kernel(CauchySolveArgs a) {
     __shared__ uint8_t exp[512];
     __shared__ uint8_t log[256];
     __shared__ uint32_t scan[kSolveThreads];
     __shared__ uint32_t s_r, s_e;
     extern __shared__ uint32_t seen[];  // n bits
-    const F f{{exp, log}};
+    using LF = typename SolveLog<GF16>::type;
+    const LF L = SolveLog<GF16>::make(exp, log, a);
     const uint64_t c = blockIdx.x;
     const uint32_t k = a.k, n = a.k + a.m, r_cap = a.rmax;
     const uint16_t *idx = a.indices + c * k;
@@ -116,44 +137,52 @@ __global__ __launch_bounds__(kSolveThreads) void cauchy_solve_kernel(CauchySolve
         if (threadIdx.x == 0) atomicOr(a.error_flag, 1u);
         return;
     }
+    // Log domain: rf, cf and q hold discrete logs; every product is a sum
+    // of independent table lookups.  Coordinates are distinct, so every
+    // difference is non-zero.
+    constexpr uint32_t O = LF::kOrder;
+    auto quot = [](uint32_t num, uint32_t den) { return (num % O + O - den % O) % O; };
     for (uint32_t i = threadIdx.x; i < r; i += kSolveThreads) {
-        uint32_t cross = 1, within = 1;
+        uint32_t cross = 0, within = 0;
         const uint32_t x = rv[i];
-        for (uint32_t h = 0; h < r; h++) cross = f.mul(cross, x ^ cv[h]);
+        for (uint32_t h = 0; h < r; h++) cross += L.lg(x ^ cv[h]);
         for (uint32_t l = 0; l < r; l++)
-            if (l != i) within = f.mul(within, x ^ rv[l]);
-        rf[i] = f.mul(cross, f.inv(within));
+            if (l != i) within += L.lg(x ^ rv[l]);
+        rf[i] = quot(cross, within);
     }
     for (uint32_t j = threadIdx.x; j < r; j += kSolveThreads) {
-        uint32_t cross = 1, within = 1;
+        uint32_t cross = 0, within = 0;
         const uint32_t x = cv[j];
-        for (uint32_t l = 0; l < r; l++) cross = f.mul(cross, x ^ rv[l]);
+        for (uint32_t l = 0; l < r; l++) cross += L.lg(x ^ rv[l]);
         for (uint32_t h = 0; h < r; h++)
-            if (h != j) within = f.mul(within, x ^ cv[h]);
-        cf[j] = f.mul(cross, f.inv(within));
+            if (h != j) within += L.lg(x ^ cv[h]);
+        cf[j] = quot(cross, within);
     }
     for (uint32_t t = threadIdx.x; t < k; t += kSolveThreads) {
         const uint32_t i = idx[t];
         if (i >= k) continue;
         const uint32_t z = a.coords[i];
-        uint32_t num = 1, den = 1;
-        for (uint32_t h = 0; h < r; h++) num = f.mul(num, z ^ cv[h]);
-        for (uint32_t l = 0; l < r; l++) den = f.mul(den, z ^ rv[l]);
-        q[t] = f.mul(num, f.inv(den));
+        uint32_t num = 0, den = 0;
+        for (uint32_t h = 0; h < r; h++) num += L.lg(z ^ cv[h]);
+        for (uint32_t l = 0; l < r; l++) den += L.lg(z ^ rv[l]);
+        q[t] = quot(num, den);
     }
     __syncthreads();
     uint16_t *fused = a.fused + c * static_cast<uint64_t>(r_cap) * k;
     for (uint32_t x = threadIdx.x; x < r * k; x += kSolveThreads) {
         const uint32_t j = x / k, t = x % k;
         const uint32_t i = idx[t];
-        uint32_t v;
+        uint32_t e, d;
         if (i >= k) {
             const uint32_t row = rpos[t];
-            v = f.mul(f.mul(rf[row], cf[j]), f.inv(cv[j] ^ rv[row]));
+            e = rf[row] + cf[j];
+            d = L.lg(cv[j] ^ rv[row]);
         } else {
-            v = f.mul(f.mul(cf[j], q[t]), f.inv(a.coords[i] ^ cv[j]));
+            e = cf[j] + q[t];
+            d = L.lg(a.coords[i] ^ cv[j]);
         }
-        fused[j * k + t] = static_cast<uint16_t>(v);
+        e = e >= d ? e - d : e + O - d;  // < 2 * O
+        fused[j * k + t] = static_cast<uint16_t>(L.ex(e));
     }
     if (threadIdx.x == 0) a.rows[c] = r;
 }
@@ -267,6 +296,32 @@ void gf8_exp_log(uint8_t exp[512], uint8_t log[256]) {
     exp[511] = exp[1];
 }
 
+hipError_t gf16_log_exp_device(const uint16_t **out) {
+    static std::mutex mu;
+    static std::map<int, DevBuf> tables;
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    std::lock_guard<std::mutex> lock(mu);
+    DevBuf &buf = tables[dev];
+    if (!buf.ptr) {
+        std::vector<uint16_t> t(kGf16LogExpEntries, 0);
+        uint16_t v = 1;
+        for (uint32_t i = 0; i < 65535u; i++) {
+            t[v] = static_cast<uint16_t>(i);
+            t[65536u + i] = t[65536u + 65535u + i] = v;
+            v = Gf16::mul(v, Gf16::kGenerator);
+        }
+        if ((err = buf.ensure(t.size() * sizeof(uint16_t))) != hipSuccess) return err;
+        if ((err = hipMemcpy(buf.ptr, t.data(), t.size() * sizeof(uint16_t), hipMemcpyHostToDevice)) != hipSuccess) {
+            buf.release();
+            return err;
+        }
+    }
+    *out = buf.as<uint16_t>();
+    return hipSuccess;
+}
+
 static hipError_t ensure_solve_tables() {
     int dev = -1;
     hipError_t err = hipGetDevice(&dev);
@@ -291,12 +346,15 @@ hipError_t launch_cauchy_solve(const CauchySolveArgs &a, bool gf16, hipStream_t 
     if (err != hipSuccess) return err;
     const size_t shmem = ((a.k + a.m + 31) / 32) * 4;
     if (shmem > 48 * 1024 || a.count > 0x7fffffffull) return hipErrorInvalidValue;
-    if (gf16)
-        hipLaunchKernelGGL(cauchy_solve_kernel<FieldGf65536>, dim3(static_cast<uint32_t>(a.count)),
-                           dim3(kSolveThreads), shmem, s, a);
-    else
-        hipLaunchKernelGGL(cauchy_solve_kernel<FieldGf256>, dim3(static_cast<uint32_t>(a.count)),
-                           dim3(kSolveThreads), shmem, s, a);
+    if (gf16) {
+        CauchySolveArgs b = a;
+        if ((err = gf16_log_exp_device(&b.gf16_log_exp)) != hipSuccess) return err;
+        hipLaunchKernelGGL(cauchy_solve_kernel<true>, dim3(static_cast<uint32_t>(a.count)), dim3(kSolveThreads),
+                           shmem, s, b);
+    } else {
+        hipLaunchKernelGGL(cauchy_solve_kernel<false>, dim3(static_cast<uint32_t>(a.count)), dim3(kSolveThreads),
+                           shmem, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -258,3 +258,45 @@ def test_decode_batch_dev_rejects_bad_indices_before_work(torch_cuda):
     with pytest.raises(DecodeError) as e:
         dec.decode_batch_dev(idx, d, r, out, 2)
     assert e.value == DecodeError("IndexOutOfRange", index=6, n=6)
+
+
+@pytest.mark.parametrize("engine", [Engine.GOOD_CAUCHY, Engine.AFFT])
+def test_encoder_shared_across_threads(torch_cuda, engine):
+    # Encoders take &self in the reference, so one handle may serve several
+    # threads at once, each with its own scratch (or none).
+    import threading
+
+    k, m, L = 12, 6, 4096
+    p = Profile.resolve(engine, k, m, L)
+    enc = BatchEncoder(p)
+    if engine == Engine.AFFT:
+        want_of = lambda d: oracle.afft_encode(k, m, L, d)  # noqa: E731
+    else:
+        c = oracle.gf256_matrix(int(engine), k, m)
+        want_of = lambda d: oracle.gf256_encode(c, k, m, L, d)  # noqa: E731
+    rng = np.random.default_rng(77)
+    inputs = [rng.integers(0, 256, k * L, dtype=np.uint8) for _ in range(6)]
+    wants = [want_of(d) for d in inputs]
+    errors = []
+
+    def worker(tid):
+        try:
+            scratch = enc.scratch() if tid % 2 else None
+            for it in range(12):
+                i = (tid + it) % len(inputs)
+                got = np.empty(m * L, np.uint8)
+                if scratch is None:
+                    enc.encode_into(inputs[i], got)
+                else:
+                    enc.encode_into_with(inputs[i], got, scratch)
+                if not np.array_equal(got, wants[i]):
+                    errors.append((tid, it))
+        except Exception as e:  # noqa: BLE001
+            errors.append((tid, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert errors == []
