@@ -16,6 +16,8 @@
 // stages.  The node tables of the pass are staged in LDS once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "afft.hpp"
 #include "afft_math.hpp"
 
@@ -69,8 +71,11 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
     return (u & ((1u << s_lo) - 1u)) | (i << s_lo) | ((u >> s_lo) << (s_lo + t));
 }
 
+// One workgroup owns a row group and `tpb` consecutive column tiles: the
+// node tables are staged once and the tiles run in turn, the next tile's
+// rows loading while the current one computes.
 template <int R>
-__global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a) {
+__global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_t tpb) {
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
     extern __shared__ uint32_t smem[];
     const uint32_t g = a.g, G = 1u << g, b_lo = a.b_lo;
@@ -120,11 +125,11 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a) {
 
     const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
     const uint64_t L = a.L;
-    const uint64_t cb = (static_cast<uint64_t>(blockIdx.x) * kTileDwords + lane) * 4u;
-    const bool valid = cb < L;
+    uint64_t cb = 0;
+    bool valid = false;
     const uint64_t z = blockIdx.z;
     const uint32_t nst = (g + RB - 1) / RB;
-    uint32_t v[R];
+    uint32_t v[R], nx[R];
 
     auto in_ptr = [&](uint32_t r) -> const uint8_t * {
         if (a.in_rows) return load_uniform(a.in_rows + r);
@@ -136,19 +141,27 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a) {
                    ? a.out_base + z * a.out_cw_stride + static_cast<uint64_t>(r - a.out_first) * L
                    : nullptr;
     };
-    auto load_global = [&](uint32_t s_lo, uint32_t t) {
+    // The raw rows of column tile `tile` in the layout of stage (s_lo, t).
+    auto load_raw = [&](uint32_t s_lo, uint32_t t, uint32_t tile, uint32_t(&dst)[R]) {
+        const uint64_t cbx = (static_cast<uint64_t>(tile) * kTileDwords + lane) * 4u;
 #pragma unroll
         for (int j = 0; j < R; j++) {
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             const uint8_t *src = in_ptr(r);
-            uint32_t w = 0;
-            if (src && valid) w = ld_col(src, cb, L);
+            dst[j] = (src && cbx < L) ? ld_col(src, cbx, L) : 0u;
+        }
+    };
+    // Copies and input scaling of the loaded tile.
+    auto prep_loaded = [&](uint32_t s_lo, uint32_t t) {
+        if (!a.copy_rows && !a.scale_in) return;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             if (a.copy_rows) {
                 uint8_t *cp = load_uniform(a.copy_rows + r);
-                if (cp && valid) st_col(cp, cb, L, w);
+                if (cp && valid) st_col(cp, cb, L, v[j]);
             }
-            if (a.scale_in) w = gmul2(sin_t + p * kTab, w);
-            v[j] = w;
+            if (a.scale_in) v[j] = gmul2(sin_t + p * kTab, v[j]);
         }
     };
     auto store_global = [&](uint32_t s_lo, uint32_t t) {
@@ -199,64 +212,80 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a) {
     auto stage_lo = [&](uint32_t k) { return k * RB; };
     auto stage_t = [&](uint32_t k) { return min(RB, g - k * RB); };
 
-    int cur;
-    if (inv) {
-        cur = 0;
-        load_global(stage_lo(0), stage_t(0));
-        for (uint32_t k = 0; k < nst; k++) {
-            if (k > 0) {
-                store_lds(stage_lo(cur), stage_t(cur));
-                __syncthreads();
-                cur = k;
-                load_lds(stage_lo(cur), stage_t(cur));
-            }
-            inv_stage(stage_lo(k), stage_t(k));
-        }
-    } else {
-        cur = nst - 1;
-        load_global(stage_lo(cur), stage_t(cur));
-    }
-    if (der) {
-        // D[p] = sum over pass bits beta clear in p of d_(b_lo+beta) * C[p | 2^beta],
-        // read from the group's coefficients in LDS.
-        store_lds(stage_lo(cur), stage_t(cur));
-        __syncthreads();
+    const uint32_t tiles = static_cast<uint32_t>((L + kTileDwords * 4 - 1) / (kTileDwords * 4));
+    const uint32_t t_first = blockIdx.x * tpb, t_end = min(tiles, t_first + tpb);
+    const uint32_t k_in = inv ? 0u : nst - 1;  // stage layout the rows load in
+    if (t_first < t_end) load_raw(stage_lo(k_in), stage_t(k_in), t_first, nx);
+    for (uint32_t tile = t_first; tile < t_end; tile++) {
+        cb = (static_cast<uint64_t>(tile) * kTileDwords + lane) * 4u;
+        valid = cb < L;
 #pragma unroll
-        for (int j = 0; j < R; j++) {
-            const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
-            uint32_t acc = 0;
-            // Unrolled over the at most 8 pass bits so the coefficient reads
-            // and table lookups of all terms are in flight together.
+        for (int j = 0; j < R; j++) v[j] = nx[j];
+        if (tile + 1 < t_end) load_raw(stage_lo(k_in), stage_t(k_in), tile + 1, nx);
+        prep_loaded(stage_lo(k_in), stage_t(k_in));
+        if (tile != t_first) __syncthreads();  // the previous tile is done with the LDS tile
+        int cur = static_cast<int>(k_in);
+        if (inv) {
+            for (uint32_t k = 0; k < nst; k++) {
+                if (k > 0) {
+                    store_lds(stage_lo(cur), stage_t(cur));
+                    __syncthreads();
+                    cur = k;
+                    load_lds(stage_lo(cur), stage_t(cur));
+                }
+                inv_stage(stage_lo(k), stage_t(k));
+            }
+        }
+        if (der) {
+            // D[p] = sum over pass bits beta clear in p of d_(b_lo+beta) * C[p | 2^beta],
+            // read from the group's coefficients in LDS.
+            store_lds(stage_lo(cur), stage_t(cur));
+            __syncthreads();
 #pragma unroll
-            for (uint32_t beta = 0; beta < 8; beta++)
-                if (beta < g && !((p >> beta) & 1u))
-                    acc ^= gmul2(der_t + (b_lo + beta) * kTab, data[(p | (1u << beta)) * kTileDwords + lane]);
-            if (a.aux_base && valid) {
-                const uint32_t r = base | (p << b_lo);
-                acc ^= ld_col(a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L, cb, L);
+            for (int j = 0; j < R; j++) {
+                const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
+                uint32_t acc = 0;
+                // Unrolled over the at most 8 pass bits so the coefficient reads
+                // and table lookups of all terms are in flight together.
+#pragma unroll
+                for (uint32_t beta = 0; beta < 8; beta++)
+                    if (beta < g && !((p >> beta) & 1u))
+                        acc ^= gmul2(der_t + (b_lo + beta) * kTab, data[(p | (1u << beta)) * kTileDwords + lane]);
+                if (a.aux_base && valid) {
+                    const uint32_t r = base | (p << b_lo);
+                    acc ^= ld_col(a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L, cb, L);
+                }
+                v[j] = acc;
             }
-            v[j] = acc;
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    if (fwd) {
-        for (int k = static_cast<int>(nst) - 1; k >= 0; k--) {
-            if (cur != k) {
-                store_lds(stage_lo(cur), stage_t(cur));
-                __syncthreads();
-                cur = k;
-                load_lds(stage_lo(cur), stage_t(cur));
+        if (fwd) {
+            for (int k = static_cast<int>(nst) - 1; k >= 0; k--) {
+                if (cur != k) {
+                    store_lds(stage_lo(cur), stage_t(cur));
+                    __syncthreads();
+                    cur = k;
+                    load_lds(stage_lo(cur), stage_t(cur));
+                }
+                fwd_stage(stage_lo(k), stage_t(k));
             }
-            fwd_stage(stage_lo(k), stage_t(k));
         }
+        store_global(stage_lo(cur), stage_t(cur));
     }
-    store_global(stage_lo(cur), stage_t(cur));
 }
 
 template <int R>
 hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
     const uint32_t G = 1u << a.g;
-    const dim3 grid(static_cast<uint32_t>((a.L + kTileDwords * 4 - 1) / (kTileDwords * 4)), 1u << (D - a.g), count);
+    const uint64_t tiles = (a.L + kTileDwords * 4 - 1) / (kTileDwords * 4);
+    const uint64_t groups = uint64_t{1} << (D - a.g);
+    // Tiles per workgroup: enough workgroups to fill every CU to its LDS
+    // capacity, no more, so each group's tables are staged as few times as
+    // possible.
+    const uint64_t per_cu = lds ? std::max<uint64_t>(1, (160u * 1024u) / lds) : 8;
+    const uint64_t want = tiles * groups * count / (256 * per_cu);
+    const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
+    const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
     static bool attr_set = false;
     if (!attr_set) {
@@ -264,7 +293,7 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(afft_pass_kernel<R>, grid, block, lds, s, a);
+    hipLaunchKernelGGL(afft_pass_kernel<R>, grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
 
