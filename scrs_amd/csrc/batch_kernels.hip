@@ -44,26 +44,31 @@ __device__ void solve_job(const BatchSolve &j, const LF &L) {
     for (uint32_t i = tid; i < r; i += nt) {
         uint32_t cross = 0, within = 0;
         const uint32_t x = rv[i];
+#pragma unroll 8
         for (uint32_t h = 0; h < r; h++) cross += L.lg(x ^ cv[h]);
-        for (uint32_t l = 0; l < r; l++)
-            if (l != i) within += L.lg(x ^ rv[l]);
+#pragma unroll 8
+        for (uint32_t l = 0; l < r; l++) within += l != i ? L.lg(x ^ rv[l]) : 0u;
         rf[i] = quot(cross, within);
         uint32_t c2 = 0, w2 = 0;
         const uint32_t y = cv[i];
+#pragma unroll 8
         for (uint32_t l = 0; l < r; l++) c2 += L.lg(y ^ rv[l]);
-        for (uint32_t h = 0; h < r; h++)
-            if (h != i) w2 += L.lg(y ^ cv[h]);
+#pragma unroll 8
+        for (uint32_t h = 0; h < r; h++) w2 += h != i ? L.lg(y ^ cv[h]) : 0u;
         cf[i] = quot(c2, w2);
     }
     for (uint32_t t = tid; t < k; t += nt) {
         if (j.slot[t] != 0xFFFFu) continue;
         const uint32_t z = j.coords[j.idx[t]];
         uint32_t num = 0, den = 0;
+#pragma unroll 8
         for (uint32_t h = 0; h < r; h++) num += L.lg(z ^ cv[h]);
+#pragma unroll 8
         for (uint32_t l = 0; l < r; l++) den += L.lg(z ^ rv[l]);
         q[t] = quot(num, den);
     }
     __syncthreads();
+#pragma unroll 4
     for (uint32_t x = tid; x < r * k; x += nt) {
         const uint32_t jj = x / k, t = x % k;
         const uint32_t s = j.slot[t];

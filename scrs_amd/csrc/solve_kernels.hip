@@ -145,17 +145,19 @@ __global__ __launch_bounds__(kSolveThreads) void cauchy_solve_kernel(CauchySolve
     for (uint32_t i = threadIdx.x; i < r; i += kSolveThreads) {
         uint32_t cross = 0, within = 0;
         const uint32_t x = rv[i];
+#pragma unroll 8
         for (uint32_t h = 0; h < r; h++) cross += L.lg(x ^ cv[h]);
-        for (uint32_t l = 0; l < r; l++)
-            if (l != i) within += L.lg(x ^ rv[l]);
+#pragma unroll 8
+        for (uint32_t l = 0; l < r; l++) within += l != i ? L.lg(x ^ rv[l]) : 0u;
         rf[i] = quot(cross, within);
     }
     for (uint32_t j = threadIdx.x; j < r; j += kSolveThreads) {
         uint32_t cross = 0, within = 0;
         const uint32_t x = cv[j];
+#pragma unroll 8
         for (uint32_t l = 0; l < r; l++) cross += L.lg(x ^ rv[l]);
-        for (uint32_t h = 0; h < r; h++)
-            if (h != j) within += L.lg(x ^ cv[h]);
+#pragma unroll 8
+        for (uint32_t h = 0; h < r; h++) within += h != j ? L.lg(x ^ cv[h]) : 0u;
         cf[j] = quot(cross, within);
     }
     for (uint32_t t = threadIdx.x; t < k; t += kSolveThreads) {
@@ -163,12 +165,15 @@ __global__ __launch_bounds__(kSolveThreads) void cauchy_solve_kernel(CauchySolve
         if (i >= k) continue;
         const uint32_t z = a.coords[i];
         uint32_t num = 0, den = 0;
+#pragma unroll 8
         for (uint32_t h = 0; h < r; h++) num += L.lg(z ^ cv[h]);
+#pragma unroll 8
         for (uint32_t l = 0; l < r; l++) den += L.lg(z ^ rv[l]);
         q[t] = quot(num, den);
     }
     __syncthreads();
     uint16_t *fused = a.fused + c * static_cast<uint64_t>(r_cap) * k;
+#pragma unroll 4
     for (uint32_t x = threadIdx.x; x < r * k; x += kSolveThreads) {
         const uint32_t j = x / k, t = x % k;
         const uint32_t i = idx[t];
