@@ -481,10 +481,12 @@ def main() -> None:
 
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
     dec_gbs = dec_bytes / (dec_ms / 1e3) / 1e9
-    traffic = None
+    traffic = traffic_enc = None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("gf256_encode_rows_terms_lds")
+        measured = json.loads(tf.read_text())
+        traffic = measured.get("gf256_decode_rows_terms_lds")
+        traffic_enc = measured.get("gf256_encode_rows_terms_lds")
     line = {
         "metric": "encode/decode GB/s (bit-exact), % HBM roofline; batched vs plain GPU; p99 latency",
         "value": round(value, 2),
@@ -508,14 +510,27 @@ def main() -> None:
         "decode_gbs": round(dec_gbs, 2),
         "encode_ms": round(enc_ms, 4),
         "decode_ms": round(dec_ms, 4),
+        # The dominant kernel is the decode's multiply-XOR (the longest launch
+        # of the step); its rate is taken over the whole decode phase (solve,
+        # tables and pointer kernels included), timed with events on the
+        # launch stream.  The encode launch is reported beside it.
         "roofline": {
             "bound": "hbm",
-            "kernel": "rows_terms_lds_kernel (encode launch)",
+            "kernel": "rows_terms_lds_kernel, decode (whole decode phase timed)",
+            "achieved": round(dec_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dec_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "roofline_encode": {
+            "bound": "hbm",
+            "kernel": "rows_terms_lds_kernel, encode",
             "achieved": round(enc_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(enc_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            "traffic": traffic_enc,
         },
         "cpu_baseline": cpu,
         "bitmatrix_kernel": bitmatrix,

@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "afft.hpp"
+#include "device.hpp"
 #include "afft_math.hpp"
 
 namespace scrs {
@@ -287,12 +288,8 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&afft_pass_kernel<R>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R>), 160 * 1024);
+    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(afft_pass_kernel<R>, grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }

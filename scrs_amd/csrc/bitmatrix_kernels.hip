@@ -32,6 +32,7 @@
 
 #include <vector>
 
+#include "device.hpp"
 #include "kernels.hpp"
 
 namespace scrs {
@@ -205,15 +206,10 @@ __global__ __launch_bounds__(256) void bitmatrix_tables_kernel(DecodePrepArgs a,
 template <int NCH>
 hipError_t launch_nch(const RowsTermsArgs &a, bool dec, dim3 grid, uint32_t tiles, uint32_t units, hipStream_t s) {
     const size_t shmem = NCH * kBmChunkV4 * sizeof(v4i);
-    // Above 64 KiB of dynamic LDS needs an explicit opt-in per kernel.
-    static const bool attrs = [] {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&bitmatrix_rows_kernel<NCH, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&bitmatrix_rows_kernel<NCH, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-        return true;
-    }();
-    (void)attrs;
+    // Above 64 KiB of dynamic LDS needs an explicit opt-in.
+    hipError_t err = dec ? lds_opt_in(reinterpret_cast<const void *>(&bitmatrix_rows_kernel<NCH, true>), 96 * 1024)
+                         : lds_opt_in(reinterpret_cast<const void *>(&bitmatrix_rows_kernel<NCH, false>), 96 * 1024);
+    if (err != hipSuccess) return err;
     if (dec)
         hipLaunchKernelGGL((bitmatrix_rows_kernel<NCH, true>), grid, dim3(kBmThreads), shmem, s, a, tiles, units);
     else

@@ -6,6 +6,8 @@
 #include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
+#include <set>
 #include <utility>
 
 namespace scrs {
@@ -72,6 +74,21 @@ struct HostBuf {
     }
     template <typename T> T *as() const { return static_cast<T *>(ptr); }
 };
+
+// Opt a kernel into `bytes` of dynamic LDS (needed above 64 KiB) on the
+// current device, once per (kernel, device).
+inline hipError_t lds_opt_in(const void *kernel, int bytes) {
+    static std::mutex mu;
+    static std::set<std::pair<const void *, int>> done;
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count({kernel, dev})) return hipSuccess;
+    err = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (err == hipSuccess) done.insert({kernel, dev});
+    return err;
+}
 
 // Makes `device` current for the scope and restores the caller's device.
 struct DeviceGuard {

@@ -18,6 +18,7 @@
 
 #include <map>
 #include <mutex>
+#include <set>
 #include <vector>
 
 #include "device.hpp"
@@ -32,7 +33,6 @@ constexpr int kSolveThreads = 256;
 
 __constant__ uint8_t c_exp8[512];
 __constant__ uint8_t c_log8[256];
-int g_solve_tables_device = -1;
 
 __device__ __forceinline__ void stage_gf8(uint8_t *exp, uint8_t *log) {
     for (int i = threadIdx.x; i < 512; i += blockDim.x) exp[i] = c_exp8[i];
@@ -327,16 +327,21 @@ hipError_t gf16_log_exp_device(const uint16_t **out) {
     return hipSuccess;
 }
 
+// The GF(2^8) constant tables are uploaded once per device (each device has
+// its own copy of the __constant__ symbols).
 static hipError_t ensure_solve_tables() {
+    static std::mutex mu;
+    static std::set<int> ready;
     int dev = -1;
     hipError_t err = hipGetDevice(&dev);
     if (err != hipSuccess) return err;
-    if (g_solve_tables_device == dev) return hipSuccess;
+    std::lock_guard<std::mutex> lock(mu);
+    if (ready.count(dev)) return hipSuccess;
     uint8_t exp[512], log[256];
     gf8_exp_log(exp, log);
     if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_exp8), exp, sizeof(exp))) != hipSuccess) return err;
     if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_log8), log, sizeof(log))) != hipSuccess) return err;
-    g_solve_tables_device = dev;
+    ready.insert(dev);
     return hipSuccess;
 }
 
