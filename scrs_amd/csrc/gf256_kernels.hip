@@ -20,6 +20,9 @@
 namespace scrs {
 namespace {
 
+#ifndef SCRS_GF8_AHEAD
+#define SCRS_GF8_AHEAD 4     // source terms per load group in the unrolled stream
+#endif
 constexpr int kThreads = 256;                     // 4 wave64 per workgroup
 constexpr int kBytesPerThread = 16;               // one 16-byte load per source
 constexpr int kTile = kThreads * kBytesPerThread; // 4096 columns per workgroup
@@ -66,7 +69,7 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
             // Fully unrolled so every table address is a compile-time LDS
             // offset.  Sources stream in groups of kAhead terms: the next
             // group's loads are in flight while the current group folds.
-            constexpr int kAhead = 4;
+            constexpr int kAhead = SCRS_GF8_AHEAD;
             constexpr int kGroups = kTermChunk / kAhead;
             const uint32_t p32 = static_cast<uint32_t>(p);
             // Uniform row base + 32-bit lane offset.
