@@ -17,6 +17,9 @@
 namespace scrs {
 namespace {
 
+#ifndef SCRS_GF16_AHEAD
+#define SCRS_GF16_AHEAD 2    // source terms per load group in the unrolled stream
+#endif
 constexpr int kThreads = 256;
 constexpr int kTile = kThreads * 16;          // bytes per workgroup tile
 constexpr int kTermChunk = 16;                // terms staged per LDS fill
@@ -26,7 +29,10 @@ constexpr uint32_t kXcds = 8;
 
 using namespace rowsdev;
 
-template <bool VEC, bool DEC, int RW>
+// FULL (8 or 16): the term count is a multiple of FULL, rows are 16-byte
+// aligned and the terms run FULL at a time fully unrolled; 0: a rolled loop
+// over chunks of kTermChunk terms.
+template <bool VEC, bool DEC, int RW, int FULL>
 __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here, uint64_t p,
                        bool do_copy) {
     constexpr int EB = 2 * RW;
@@ -39,8 +45,9 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
     const bool active = p < a.L;
     const uint8_t *tab_cw = reinterpret_cast<const uint8_t *>(a.tables + c * a.table_cw_stride);
     const uint8_t *src_cw = DEC ? nullptr : a.src_base + c * a.src_cw_stride;
-    for (uint32_t t0 = 0; t0 < a.T; t0 += kTermChunk) {
-        const uint32_t tc = min(static_cast<uint32_t>(kTermChunk), a.T - t0);
+    constexpr uint32_t kChunk = FULL ? FULL : kTermChunk;
+    for (uint32_t t0 = 0; t0 < a.T; t0 += kChunk) {
+        const uint32_t tc = min(kChunk, a.T - t0);
         __syncthreads();
         // Stage: entry (q, n) of term t keeps its first EB bytes (rows 0..RW-1).
         for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += kThreads) {
@@ -52,6 +59,61 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
         }
         __syncthreads();
         if (!active) continue;
+        if constexpr (FULL != 0) {
+            // The chunk is unrolled so that every table address is a
+            // compile-time LDS offset, and sources stream in groups of
+            // kAhead terms with the next group's loads in flight.
+            constexpr int kAhead = SCRS_GF16_AHEAD;
+            constexpr int kGroups = FULL / kAhead;
+            static_assert(FULL % (2 * kAhead) == 0, "groups run in pairs");
+            const uint32_t p32 = static_cast<uint32_t>(p);
+            auto row_of = [&](int t) -> const uint8_t * {
+                if (DEC) return load_uniform(a.src_ptrs + c * a.T + t0 + t);
+                return src_cw + static_cast<uint64_t>(t0 + t) * a.L;
+            };
+            auto ld = [&](int t, uint32_t(&w)[4]) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(row_of(t) + p32);
+                w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+            };
+            // Present data rows are written verbatim as they stream by: the
+            // store's buffer range is L for those terms and zero otherwise,
+            // so the hardware drops the other stores without a branch.
+            const uint32_t copy_range = do_copy ? static_cast<uint32_t>(a.L) : 0u;
+            auto cp = [&](int t, const uint32_t(&w)[4]) {
+                if (!DEC) return;
+                uint8_t *dst = load_uniform(a.copy_ptrs + c * a.T + t0 + t);
+                const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+                    dst ? dst : const_cast<uint8_t *>(row_of(t)), static_cast<short>(0), dst ? copy_range : 0u,
+                    0x00020000);
+                const v4u32 v = {w[0], w[1], w[2], w[3]};
+                __builtin_amdgcn_raw_buffer_store_b128(v, r, p32, 0, 0);
+            };
+            uint32_t wa[kAhead][4], wb[kAhead][4];
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) ld(u, wa[u]);
+#pragma unroll
+            for (int g = 0; g < kGroups; g += 2) {
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) ld((g + 1) * kAhead + u, wb[u]);
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) {
+                    cp(g * kAhead + u, wa[u]);
+                    fold_term16<EB>(acc, lds + (g * kAhead + u) * 64 * EB, wa[u]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (g + 2 < kGroups) {
+#pragma unroll
+                    for (int u = 0; u < kAhead; u++) ld((g + 2) * kAhead + u, wa[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) {
+                    cp((g + 1) * kAhead + u, wb[u]);
+                    fold_term16<EB>(acc, lds + ((g + 1) * kAhead + u) * 64 * EB, wb[u]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            continue;
+        } else {
 #pragma unroll 1
         for (uint32_t t = 0; t < tc; t++) {
             const uint64_t gt = t0 + t;
@@ -63,6 +125,7 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
                 if (cp) store16<VEC>(cp, p, a.L, w);
             }
             fold_term16<EB>(acc, lds + t * 64 * EB, w);
+        }
         }
     }
     if (!active) return;
@@ -99,7 +162,7 @@ __device__ void copy_only16(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
     }
 }
 
-template <bool VEC, bool DEC>
+template <bool VEC, bool DEC, int FULL>
 __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a, uint32_t tiles, uint32_t units) {
     __shared__ __align__(16) uint8_t lds[kTermChunk * 64 * 32];
     // XCD-aware: all tiles of one (codeword, row group) on one XCD.
@@ -118,16 +181,18 @@ __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a,
     if (DEC && rows_here == 0)
         copy_only16<VEC>(a, c, p);
     else if (rows_here <= 4)
-        pass16<VEC, DEC, 4>(a, lds, c, rg, rows_here, p, do_copy);
+        pass16<VEC, DEC, 4, FULL>(a, lds, c, rg, rows_here, p, do_copy);
     else if (rows_here <= 8)
-        pass16<VEC, DEC, 8>(a, lds, c, rg, rows_here, p, do_copy);
+        pass16<VEC, DEC, 8, FULL>(a, lds, c, rg, rows_here, p, do_copy);
     else
-        pass16<VEC, DEC, 16>(a, lds, c, rg, rows_here, p, do_copy);
+        pass16<VEC, DEC, 16, FULL>(a, lds, c, rg, rows_here, p, do_copy);
 }
 
 }  // namespace
 
 hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
+    // The 16-byte paths address a row with a 32-bit column offset.
+    if (args.L > 0xffffffffull) vec16 = false;
     if (args.count == 0 || args.L == 0) return hipSuccess;
     RowsTermsArgs a = args;
     const uint64_t tiles = (a.L + kTile - 1) / kTile;
@@ -143,14 +208,24 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
     if (dec && a.accumulate) return hipErrorInvalidValue;
     const dim3 grid(static_cast<uint32_t>(blocks));
     const uint32_t t32 = static_cast<uint32_t>(tiles), u32 = static_cast<uint32_t>(units);
-    if (vec16 && dec)
-        hipLaunchKernelGGL((rows_terms16_kernel<true, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    // Term counts that are multiples of 16 or 8 take the unrolled kernels.
+    const int full = !vec16 ? 0 : (a.T % 16 == 0 ? 16 : (a.T % 8 == 0 ? 8 : 0));
+    if (full == 16 && dec)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, true, 16>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else if (full == 16)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, false, 16>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else if (full == 8 && dec)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, true, 8>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else if (full == 8)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, false, 8>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    else if (vec16 && dec)
+        hipLaunchKernelGGL((rows_terms16_kernel<true, true, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (vec16)
-        hipLaunchKernelGGL((rows_terms16_kernel<true, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
+        hipLaunchKernelGGL((rows_terms16_kernel<true, false, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (dec)
-        hipLaunchKernelGGL((rows_terms16_kernel<false, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
+        hipLaunchKernelGGL((rows_terms16_kernel<false, true, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else
-        hipLaunchKernelGGL((rows_terms16_kernel<false, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
+        hipLaunchKernelGGL((rows_terms16_kernel<false, false, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     return hipGetLastError();
 }
 

@@ -221,6 +221,8 @@ __global__ void fill_splitmix_kernel(uint8_t *dst, uint64_t len, uint64_t seed, 
 }  // namespace
 
 hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
+    // The 16-byte paths address a row with a 32-bit column offset.
+    if (args.L > 0xffffffffull) vec16 = false;
     if (args.count == 0 || args.L == 0) return hipSuccess;
     RowsTermsArgs a = args;
     const uint64_t tiles = (a.L + kTile - 1) / kTile;

@@ -186,14 +186,30 @@ __device__ __forceinline__ void fold_term16(uint32_t (&acc)[8][EB / 4], const ui
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const uint32_t x = (w[j] >> (16 * h)) & 0xffffu;
-            uint32_t e0[EB / 4], e1[EB / 4], e2[EB / 4], e3[EB / 4];
-            load_entry<EB>(tab, ((x & 15u) << SH), e0);
-            load_entry<EB>(tab, Q + (((x >> 4) & 15u) << SH), e1);
-            load_entry<EB>(tab, 2 * Q + (((x >> 8) & 15u) << SH), e2);
-            load_entry<EB>(tab, 3 * Q + ((x >> 12) << SH), e3);
             uint32_t(&a)[EB / 4] = acc[2 * j + h];
+            if constexpr (EB == 32) {
+                // Two lookups in flight at a time keep the register budget
+                // of the 16-row accumulators at four waves per SIMD.
+                uint32_t e0[8], e1[8];
+                load_entry<EB>(tab, ((x & 15u) << SH), e0);
+                load_entry<EB>(tab, Q + (((x >> 4) & 15u) << SH), e1);
 #pragma unroll
-            for (int d = 0; d < EB / 4; d++) a[d] = xor3(xor3(a[d], e0[d], e1[d]), e2[d], e3[d]);
+                for (int d = 0; d < 8; d++) a[d] = xor3(a[d], e0[d], e1[d]);
+                __builtin_amdgcn_sched_barrier(0);
+                load_entry<EB>(tab, 2 * Q + (((x >> 8) & 15u) << SH), e0);
+                load_entry<EB>(tab, 3 * Q + ((x >> 12) << SH), e1);
+#pragma unroll
+                for (int d = 0; d < 8; d++) a[d] = xor3(a[d], e0[d], e1[d]);
+                __builtin_amdgcn_sched_barrier(0);
+            } else {
+                uint32_t e0[EB / 4], e1[EB / 4], e2[EB / 4], e3[EB / 4];
+                load_entry<EB>(tab, ((x & 15u) << SH), e0);
+                load_entry<EB>(tab, Q + (((x >> 4) & 15u) << SH), e1);
+                load_entry<EB>(tab, 2 * Q + (((x >> 8) & 15u) << SH), e2);
+                load_entry<EB>(tab, 3 * Q + ((x >> 12) << SH), e3);
+#pragma unroll
+                for (int d = 0; d < EB / 4; d++) a[d] = xor3(xor3(a[d], e0[d], e1[d]), e2[d], e3[d]);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     }

@@ -308,3 +308,38 @@ def test_streaming_geometry_200_50_8192(torch_cuda):
         got = np.zeros(k * L, np.uint8)
         dec.finalize_into_with(got, scratch)
         assert np.array_equal(got, d)
+
+
+@pytest.mark.parametrize("k,m,L,count", [
+    (48, 24, 1024, 3),   # 16-term unrolled chunks, a 16-row and an 8-row group
+    (32, 5, 4096, 2),    # 16-term chunks, one 8-row pass
+    (40, 19, 2048, 2),   # 8-term chunks, a 16-row and a 4-row group
+    (24, 3, 512, 4),     # 8-term chunks, one 4-row pass
+    (36, 6, 512, 2),     # neither: the rolled loop
+])
+def test_batch_round_trip_term_chunks(torch_cuda, k, m, L, count):
+    """Device batch encode and decode at term counts that take each chunking
+    of the multiply-XOR (multiples of 16, of 8, neither), against the oracle
+    for every codeword."""
+    torch = torch_cuda
+    from scrs_amd.codec import fill_random_dev
+    p = Profile.resolve(TOWER, k, m, L)
+    data = torch.empty(count * k * L, dtype=torch.uint8, device="cuda")
+    fill_random_dev(data, data.numel(), k * 1000 + m)
+    rep = torch.empty(count * m * L, dtype=torch.uint8, device="cuda")
+    IncrementalEncoder(p).encode_batch_dev(data, rep, count)
+    rng = np.random.default_rng(k + m + L)
+    idx = np.zeros((count, k), np.uint16)
+    for c in range(count):
+        r = int(rng.integers(1, m + 1))
+        erased = set(rng.choice(k, size=r, replace=False).tolist())
+        idx[c] = rng.permutation([i for i in range(k + m) if i not in erased])[:k]
+    out = torch.full_like(data, 0xAB)
+    BatchDecoder(p).decode_batch_dev(idx, data, rep, out, count)
+    torch.cuda.synchronize()
+    cmat = oracle.tower_cauchy_matrix(k, m)
+    d_all, r_all = data.cpu().numpy(), rep.cpu().numpy()
+    for c in range(count):
+        d = d_all[c * k * L:(c + 1) * k * L]
+        assert np.array_equal(r_all[c * m * L:(c + 1) * m * L], oracle.tower_encode(cmat, k, m, L, d)), c
+    assert torch.equal(out, data)
