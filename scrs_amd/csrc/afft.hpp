@@ -52,6 +52,8 @@ struct AfftPassArgs {
     uint32_t inv_base = 0, fwd_base = 0;   // point of buffer row 0, per direction
     uint32_t flags = 0;
     uint32_t row_limit = 0;                // only rows < row_limit are needed downstream
+    // Every pointer in in_rows / copy_rows / out_rows is a multiple of 4.
+    bool rows_aligned4 = false;
 };
 
 // Launch one pass over all 2^D rows (groups = 2^(D-g)) for `count`

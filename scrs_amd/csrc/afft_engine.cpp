@@ -331,6 +331,10 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
     base.node_tabs = dev->node_tabs.as<uint16_t>();
     base.deriv_tabs = dev->deriv_tabs.as<uint16_t>();
     base.L = L;
+    // Row tables point into the data, repair and output buffers at multiples
+    // of L (caller pointers from `ptrs` are not inspected).
+    auto al4 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; };
+    base.rows_aligned4 = !ptrs && L % 4 == 0 && al4(data_base) && al4(repair_base) && al4(out);
     const uint8_t *const *rows_in = in_rows.as<const uint8_t *const>();
     uint8_t *const *rows_copy = copy_rows.as<uint8_t *const>();
     uint8_t *const *rows_out = out_rows.as<uint8_t *const>();

@@ -35,6 +35,9 @@ __device__ __forceinline__ T *load_uniform(T *const *p) {
 }
 
 __device__ __forceinline__ uint32_t gmul(const uint16_t *t, uint32_t x) {
+    // Tables are 128-byte aligned in LDS, so an entry address is the table
+    // base OR the entry offset.
+    t = static_cast<const uint16_t *>(__builtin_assume_aligned(t, 128));
     return static_cast<uint32_t>(t[x & 15u]) ^ t[16u | ((x >> 4) & 15u)] ^ t[32u | ((x >> 8) & 15u)] ^
            t[48u | ((x >> 12) & 15u)];
 }
@@ -63,6 +66,23 @@ __device__ __forceinline__ void st_col(uint8_t *p, uint64_t cb, uint64_t L, uint
         if (cb + b < L) p[cb + b] = static_cast<uint8_t>(w >> (8 * b));
 }
 
+// A4 (every row address and L a multiple of 4, L < 2^32): dword columns
+// through buffer instructions.  A null row or a column at or past L is a
+// zero load and a dropped store by the buffer's range, with no branch; the
+// branchy byte paths above are kept for unaligned rows.
+typedef uint32_t v1u32 __attribute__((ext_vector_type(1)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const uint8_t *row, const void *dummy, uint64_t L) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(row ? row : static_cast<const uint8_t *>(dummy)),
+                                             static_cast<short>(0), row ? static_cast<uint32_t>(L) : 0u,
+                                             0x00020000);
+}
+__device__ __forceinline__ uint32_t ld_col4(const uint8_t *row, const void *dummy, uint32_t cb, uint64_t L) {
+    return __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row, dummy, L), cb, 0, 0);
+}
+__device__ __forceinline__ void st_col4(uint8_t *row, const void *dummy, uint32_t cb, uint64_t L, uint32_t w) {
+    __builtin_amdgcn_raw_buffer_store_b32(w, row_rsrc(row, dummy, L), cb, 0, 0);
+}
+
 // Local row of register j in a stage over local bits [s_lo, s_lo + t) for
 // wave `wv` holding R rows: tile u = wv*(R >> t) + (j >> t), i = j & (2^t-1).
 template <int R>
@@ -75,7 +95,7 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 // One workgroup owns a row group and `tpb` consecutive column tiles: the
 // node tables are staged once and the tiles run in turn, the next tile's
 // rows loading while the current one computes.
-template <int R>
+template <int R, bool A4>
 __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_t tpb) {
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
     extern __shared__ uint32_t smem[];
@@ -149,7 +169,10 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
         for (int j = 0; j < R; j++) {
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             const uint8_t *src = in_ptr(r);
-            dst[j] = (src && cbx < L) ? ld_col(src, cbx, L) : 0u;
+            if constexpr (A4)
+                dst[j] = ld_col4(src, a.node_tabs, static_cast<uint32_t>(cbx), L);
+            else
+                dst[j] = (src && cbx < L) ? ld_col(src, cbx, L) : 0u;
         }
     };
     // Copies and input scaling of the loaded tile.
@@ -160,7 +183,10 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             if (a.copy_rows) {
                 uint8_t *cp = load_uniform(a.copy_rows + r);
-                if (cp && valid) st_col(cp, cb, L, v[j]);
+                if constexpr (A4)
+                    st_col4(cp, a.node_tabs, static_cast<uint32_t>(cb), L, v[j]);
+                else if (cp && valid)
+                    st_col(cp, cb, L, v[j]);
             }
             if (a.scale_in) v[j] = gmul2(sin_t + p * kTab, v[j]);
         }
@@ -170,7 +196,11 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
         for (int j = 0; j < R; j++) {
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             uint8_t *dst = out_ptr(r);
-            if (dst && valid) st_col(dst, cb, L, a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
+            if constexpr (A4)
+                st_col4(dst, a.node_tabs, static_cast<uint32_t>(cb), L,
+                        a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
+            else if (dst && valid)
+                st_col(dst, cb, L, a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
         }
     };
     auto load_lds = [&](uint32_t s_lo, uint32_t t) {
@@ -252,9 +282,13 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
                 for (uint32_t beta = 0; beta < 8; beta++)
                     if (beta < g && !((p >> beta) & 1u))
                         acc ^= gmul2(der_t + (b_lo + beta) * kTab, data[(p | (1u << beta)) * kTileDwords + lane]);
-                if (a.aux_base && valid) {
+                if (a.aux_base && (A4 || valid)) {
                     const uint32_t r = base | (p << b_lo);
-                    acc ^= ld_col(a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L, cb, L);
+                    const uint8_t *row = a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L;
+                    if constexpr (A4)
+                        acc ^= ld_col4(row, a.node_tabs, static_cast<uint32_t>(cb), L);
+                    else
+                        acc ^= ld_col(row, cb, L);
                 }
                 v[j] = acc;
             }
@@ -275,7 +309,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     }
 }
 
-template <int R>
+template <int R, bool A4>
 hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
     const uint32_t G = 1u << a.g;
     const uint64_t tiles = (a.L + kTileDwords * 4 - 1) / (kTileDwords * 4);
@@ -288,9 +322,9 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R>), 160 * 1024);
+    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4>), 160 * 1024);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(afft_pass_kernel<R>, grid, block, lds, s, a, tpb);
+    hipLaunchKernelGGL((afft_pass_kernel<R, A4>), grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
 
@@ -424,11 +458,25 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
         return hipErrorInvalidValue;
     const size_t lds = afft_pass_lds_bytes(a.g, a.flags, a.scale_in != nullptr, a.scale_out != nullptr);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
+    // Dword buffer instructions when every row the pass touches starts on a
+    // multiple of 4 (row tables: as declared by the caller).
+    auto al4 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; };
+    const bool tables = a.in_rows || a.copy_rows || a.out_rows;
+    const bool a4 = a.L % 4 == 0 && a.L <= 0xffffffffull && al4(a.in_base) && al4(a.out_base) && al4(a.aux_base) &&
+                    (a.in_cw_stride | a.out_cw_stride | a.aux_cw_stride) % 4 == 0 && (!tables || a.rows_aligned4);
+    if (a4) {
+        switch (a.g) {
+            case 1: return launch_pass_r<2, true>(a, D, count, lds, s);
+            case 2: return launch_pass_r<4, true>(a, D, count, lds, s);
+            case 3: return launch_pass_r<8, true>(a, D, count, lds, s);
+            default: return launch_pass_r<16, true>(a, D, count, lds, s);
+        }
+    }
     switch (a.g) {
-        case 1: return launch_pass_r<2>(a, D, count, lds, s);
-        case 2: return launch_pass_r<4>(a, D, count, lds, s);
-        case 3: return launch_pass_r<8>(a, D, count, lds, s);
-        default: return launch_pass_r<16>(a, D, count, lds, s);
+        case 1: return launch_pass_r<2, false>(a, D, count, lds, s);
+        case 2: return launch_pass_r<4, false>(a, D, count, lds, s);
+        case 3: return launch_pass_r<8, false>(a, D, count, lds, s);
+        default: return launch_pass_r<16, false>(a, D, count, lds, s);
     }
 }
 
