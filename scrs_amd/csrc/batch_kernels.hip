@@ -167,10 +167,10 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
     for (uint32_t z = blockIdx.x; z < hdr->nzero; z += gridDim.x) {
         const uint64_t addr = hdr->zero[2 * z], len = hdr->zero[2 * z + 1];
         if (((addr | len) & 15) == 0) {
-            uint4 *dst = reinterpret_cast<uint4 *>(addr);
-            for (uint64_t i = threadIdx.x; i < len / 16; i += blockDim.x) dst[i] = make_uint4(0, 0, 0, 0);
+            SCRS_GLOBAL v4u32 *dst = gbl_mut<v4u32>(reinterpret_cast<void *>(addr));
+            for (uint64_t i = threadIdx.x; i < len / 16; i += blockDim.x) dst[i] = v4u32{0, 0, 0, 0};
         } else {  // dword-aligned rows, the only other kind a split range has
-            uint32_t *dst = reinterpret_cast<uint32_t *>(addr);
+            SCRS_GLOBAL uint32_t *dst = gbl_mut<uint32_t>(reinterpret_cast<void *>(addr));
             for (uint64_t i = threadIdx.x; i < len / 4; i += blockDim.x) dst[i] = 0;
         }
     }
@@ -183,7 +183,7 @@ __global__ void batch_reset_kernel(uint32_t *counter) {
 // ---- multiply-XOR ----------------------------------------------------------
 
 template <bool VEC, bool DEC, int RW, int TPU>
-__device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here, uint64_t p,
+__device__ __forceinline__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here, uint64_t p,
                       uint32_t tb, uint32_t te, uint8_t *lds) {
     constexpr uint32_t D = RW / 4;
     const bool active = p < R.L;
@@ -203,7 +203,8 @@ __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t 
                 for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += TPU) {
                     const uint32_t t = i / (32 * D), rem = i % (32 * D);
                     const uint32_t n = rem / D, d = rem % D;
-                    reinterpret_cast<uint32_t *>(lds)[i] = R.tables[((t0 + t) * R.groups + rg) * 128 + n * 4 + d];
+                    reinterpret_cast<uint32_t *>(lds)[i] =
+                        *gbl<uint32_t>(R.tables + ((t0 + t) * R.groups + rg) * 128 + n * 4 + d);
                 }
                 __syncthreads();
             }
@@ -238,7 +239,7 @@ __device__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t 
 }
 
 template <bool VEC, bool DEC, int RW, int TPU>
-__device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here,
+__device__ __forceinline__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here,
                        uint64_t p, uint32_t tb, uint32_t te, uint8_t *lds) {
     constexpr int EB = 2 * RW;
     constexpr uint32_t D = EB / 4;
@@ -261,8 +262,7 @@ __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t
                     const uint32_t t = i / (64 * D), rem = i % (64 * D);
                     const uint32_t e = rem / D, d = rem % D;
                     reinterpret_cast<uint32_t *>(lds)[t * 64 * D + lds16_index<EB>(e, d)] =
-                        *reinterpret_cast<const uint32_t *>(
-                            tb + (static_cast<uint64_t>(t0 + t) * R.groups + rg) * 2048 + e * 32 + d * 4);
+                        *gbl<uint32_t>(tb + (static_cast<uint64_t>(t0 + t) * R.groups + rg) * 2048 + e * 32 + d * 4);
                 }
                 __syncthreads();
             }
@@ -297,7 +297,7 @@ __device__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t
 
 // Decode with no missing data row: only the present rows are copied.
 template <bool VEC>
-__device__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint64_t p, uint32_t tb, uint32_t te) {
+__device__ __forceinline__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint64_t p, uint32_t tb, uint32_t te) {
     if (p >= R.L) return;
     for (uint32_t c = cw0; c < cw0 + ncw; c++)
         for (uint32_t t = tb; t < te; t++) {
@@ -310,7 +310,7 @@ __device__ void unit_copy(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint6
 }
 
 template <bool VEC, bool DEC, int TPU>
-__device__ void run_unit(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t seg, uint64_t p,
+__device__ __forceinline__ void run_unit(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t seg, uint64_t p,
                          uint8_t *lds) {
     const uint32_t base = rg * 16;
     const uint32_t rows_here = R.rows > base ? min(16u, R.rows - base) : 0u;
@@ -330,6 +330,18 @@ __device__ void run_unit(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32
     }
 }
 
+// A range read through the constant address space: scalar loads, so its
+// fields stay in SGPRs for the whole unit.
+__device__ __forceinline__ BatchRange load_range(const BatchRange *p) {
+    static_assert(sizeof(BatchRange) % 4 == 0, "whole dwords");
+    BatchRange r;
+    const_u32_ptr s = (const_u32_ptr)(p);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&r);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(BatchRange) / 4; i++) d[i] = s[i];
+    return r;
+}
+
 template <int TPU>
 __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr, uint32_t *counter) {
     __shared__ __align__(16) uint8_t lds[kLds];
@@ -345,10 +357,11 @@ __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr,
         uint32_t lo = 0, hi = nranges - 1;
         while (lo < hi) {
             const uint32_t mid = (lo + hi + 1) >> 1;
-            if (ranges[mid].unit_base <= u) lo = mid;
+            if (load_uniform(&ranges[mid].unit_base) <= u) lo = mid;
             else hi = mid - 1;
         }
-        const BatchRange R = ranges[lo];
+        // The search is wave-uniform; saying so keeps the range in SGPRs.
+        const BatchRange R = load_range(ranges + __builtin_amdgcn_readfirstlane(lo));
         const uint32_t local = u - R.unit_base, per = R.groups * R.tiles * R.segs;
         const uint32_t chunk = local / per, rem = local % per;
         const uint32_t seg = rem % R.segs, rt = rem / R.segs;

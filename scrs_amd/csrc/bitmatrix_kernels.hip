@@ -34,6 +34,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "rows_device.hpp"
 
 namespace scrs {
 namespace {
@@ -49,7 +50,7 @@ constexpr int kBmChunkV4 = 8 * 64;       // 16-byte operands per 8-source chunk 
 constexpr uint32_t kXcds = 8;
 
 __device__ __forceinline__ uint32_t ld_dword(const uint8_t *p) {
-    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+    return __builtin_nontemporal_load(rowsdev::gbl<uint32_t>(p));
 }
 
 // NCH 8-source chunks (T <= 8 * NCH).  DEC: decode addressing (pointer
@@ -124,8 +125,8 @@ __global__ __launch_bounds__(kBmThreads, 2) void bitmatrix_rows_kernel(RowsTerms
 #pragma unroll
             for (int ch = 0; ch < NCH; ch++) {
                 uint8_t *c0 = s_copy[8 * ch + 2 * b], *c1 = s_copy[8 * ch + 2 * b + 1];
-                if (c0) *reinterpret_cast<uint32_t *>(c0 + col) = xa[2 * ch];
-                if (c1) *reinterpret_cast<uint32_t *>(c1 + col) = xa[2 * ch + 1];
+                if (c0) *rowsdev::gbl_mut<uint32_t>(c0 + col) = xa[2 * ch];
+                if (c1) *rowsdev::gbl_mut<uint32_t>(c1 + col) = xa[2 * ch + 1];
             }
         }
         if (rows_here) {
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(kBmThreads, 2) void bitmatrix_rows_kernel(RowsTerms
                     o |= ((g01 | g23) >> (7 - i)) & (0x01010101u << i);
                 }
                 uint8_t *dst = s_dst[4 * b + v];
-                if (active && dst) *reinterpret_cast<uint32_t *>(dst + col) = o;
+                if (active && dst) *rowsdev::gbl_mut<uint32_t>(dst + col) = o;
             }
         }
 #pragma unroll

@@ -72,7 +72,7 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
                 return src_cw + static_cast<uint64_t>(t0 + t) * a.L;
             };
             auto ld = [&](int t, uint32_t(&w)[4]) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(row_of(t) + p32);
+                const v4u32 v = *gbl<v4u32>(row_of(t) + p32);
                 w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
             };
             // Present data rows are written verbatim as they stream by: the

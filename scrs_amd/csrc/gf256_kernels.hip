@@ -77,6 +77,11 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
                 if (DEC) return load_uniform(a.src_ptrs + c * a.T + t0 + t);
                 return src_cw + static_cast<uint64_t>(t0 + t) * a.L;
             };
+            // Generic (flat) loads on purpose: a flat load also counts
+            // against the LDS wait counter, so a wave's first table lookup
+            // waits for its prefetched sources.  That throttling measured
+            // 1.5% faster on the HBM-bound decode than global loads, and
+            // the same on encode.
             auto ld = [&](int t, uint32_t(&w)[4]) {
                 const uint4 v = *reinterpret_cast<const uint4 *>(row_of(t) + p32);
                 w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;

@@ -10,6 +10,23 @@
 namespace scrs {
 namespace rowsdev {
 
+// Payload rows always live in global memory (device or mapped host
+// allocations).  Pointers read from tables are generic to the compiler,
+// which would issue flat instructions for them; a flat access counts
+// against the LDS wait counter too, so a wait for a table lookup would also
+// wait for every source load still in flight.  Accesses go through
+// global-address-space pointers instead.
+#define SCRS_GLOBAL __attribute__((address_space(1)))
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ const SCRS_GLOBAL T *gbl(const void *p) {
+    return (const SCRS_GLOBAL T *)(p);
+}
+template <typename T>
+__device__ __forceinline__ SCRS_GLOBAL T *gbl_mut(void *p) {
+    return (SCRS_GLOBAL T *)(p);
+}
+
 // 16 bytes at column p of a row (p < L, p a multiple of 16).  VEC: the row
 // address and L are multiples of 16, known at compile time.  Otherwise the
 // width follows the row's alignment at run time (row pointers are
@@ -21,16 +38,16 @@ namespace rowsdev {
 template <bool VEC>
 __device__ __forceinline__ void load16(const uint8_t *src, uint64_t p, uint64_t L, uint32_t (&w)[4]) {
     if (VEC || (((reinterpret_cast<uint64_t>(src) | L) & 15) == 0)) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(src + p);
+        const v4u32 v = *gbl<v4u32>(src + p);
         w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
     } else {
         const uint8_t *at = src + p;
         const uint32_t sh = static_cast<uint32_t>(reinterpret_cast<uint64_t>(at) & 3u);
-        const uint32_t *al = reinterpret_cast<const uint32_t *>(at - sh);
+        const uint8_t *al = at - sh;
         const uint8_t *end = src + L;
         uint32_t d[5];
 #pragma unroll
-        for (int i = 0; i < 5; i++) d[i] = reinterpret_cast<const uint8_t *>(al + i) < end ? al[i] : 0u;
+        for (int i = 0; i < 5; i++) d[i] = al + 4 * i < end ? *gbl<uint32_t>(al + 4 * i) : 0u;
 #pragma unroll
         for (int j = 0; j < 4; j++) w[j] = __builtin_amdgcn_alignbyte(d[j + 1], d[j], sh);
     }
@@ -47,7 +64,7 @@ __device__ __noinline__ void store16_bytes(uint8_t *dst, uint64_t p, uint64_t L,
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const uint64_t q = p + j * 4 + b;
-            if (q < L) dst[q] = static_cast<uint8_t>(w[j] >> (8 * b));
+            if (q < L) *gbl_mut<uint8_t>(dst + q) = static_cast<uint8_t>(w[j] >> (8 * b));
         }
 }
 
@@ -57,11 +74,11 @@ template <bool VEC>
 __device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
     const uint64_t al = reinterpret_cast<uint64_t>(dst) | L;
     if (VEC || (al & 15) == 0) {
-        *reinterpret_cast<uint4 *>(dst + p) = make_uint4(w[0], w[1], w[2], w[3]);
+        *gbl_mut<v4u32>(dst + p) = v4u32{w[0], w[1], w[2], w[3]};
     } else if ((al & 3) == 0) {
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            if (p + 4 * j < L) *reinterpret_cast<uint32_t *>(dst + p + 4 * j) = w[j];
+            if (p + 4 * j < L) *gbl_mut<uint32_t>(dst + p + 4 * j) = w[j];
     } else {
         store16_bytes(dst, p, L, w[0], w[1], w[2], w[3]);
     }
@@ -70,10 +87,11 @@ __device__ __forceinline__ void store16(uint8_t *dst, uint64_t p, uint64_t L, co
 // XOR 16 bytes into a row whose address and L are multiples of 4 (term-split
 // partial sums): one atomic per dword inside the row.
 __device__ __forceinline__ void xor16_atomic(uint8_t *dst, uint64_t p, uint64_t L, const uint32_t (&w)[4]) {
-    uint32_t *d = reinterpret_cast<uint32_t *>(dst + p);
 #pragma unroll
     for (int j = 0; j < 4; j++)
-        if (p + 4 * j < L) atomicXor(d + j, w[j]);
+        if (p + 4 * j < L)
+            __hip_atomic_fetch_xor(gbl_mut<uint32_t>(dst + p + 4 * j), w[j], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Entry of RW bytes at byte offset `off` of the LDS chunk.
@@ -128,8 +146,6 @@ __device__ __forceinline__ void fold_term(uint32_t (&acc)[16][RW / 4], const uin
         __builtin_amdgcn_sched_barrier(0);
     }
 }
-
-typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
 // Entries of the per-codeword pointer / row-count tables are the same for
 // every lane and the kernel never writes those tables: read them through the
