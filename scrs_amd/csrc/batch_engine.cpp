@@ -123,7 +123,8 @@ scrs_status BatchEngine::validate(const scrs_job &j) const {
     if (j.op != SCRS_JOB_DECODE) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
     if (!j.indices || !j.payloads || !j.dst) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     const uint64_t k = p.k, n = p.k + p.m;
-    std::vector<uint64_t> seen((n + 63) / 64, 0);
+    std::vector<uint64_t> &seen = seen_;
+    seen.assign((n + 63) / 64, 0);
     for (uint64_t t = 0; t < k; t++) {
         const uint64_t i = j.indices[t];
         if (i >= n) return st(SCRS_DECODE_INDEX_OUT_OF_RANGE, i, n);
@@ -265,21 +266,27 @@ CacheEntry *BatchEngine::encoder_tables(const scrs_profile &p, Plan &plan, scrs_
 scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p) {
     p = Plan();
     // Encode jobs of one geometry, length and alignment class share a range
-    // (class 2: rows 16-byte aligned; 1: destination rows 4-byte aligned).
-    std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t, int>, std::vector<uint64_t>> groups;
-    std::vector<uint64_t> decodes;
+    // (class 2: rows 16-byte aligned; 1: destination rows 4-byte aligned):
+    // sorted by that key, each run of equal keys is one range, in job order.
+    enc_keys_.clear();
+    decodes_.clear();
+    uint64_t nptrs = 0;
     for (uint64_t i = 0; i < count; i++) {
         const scrs_job &j = jobs[i];
+        const uint64_t L = j.profile.symbol_len;
         if (j.op == SCRS_JOB_ENCODE) {
-            const uint64_t L = j.profile.symbol_len;
             const int v = (L % 16 == 0 && aligned16(j.src) && aligned16(j.dst)) ? 2
                           : (L % 4 == 0 && aligned4(j.dst))                     ? 1
                                                                                 : 0;
-            groups[std::make_tuple(j.profile.engine, j.profile.k, j.profile.m, L, v)].push_back(i);
+            enc_keys_.push_back({j.profile.engine, v, j.profile.k, j.profile.m, L, i});
+            nptrs += j.profile.k + j.profile.m;
         } else {
-            decodes.push_back(i);
+            decodes_.push_back(i);
+            nptrs += 3 * j.profile.k;
         }
     }
+    std::sort(enc_keys_.begin(), enc_keys_.end());
+    p.ptrs.reserve(nptrs);
     uint64_t units = 0;
     auto add_range = [&](BatchRange r, const Plan::RangeRef &ref) -> bool {
         r.segs = 1;
@@ -290,8 +297,13 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
         p.rref.push_back(ref);
         return units <= 0xffffffffull;
     };
-    for (auto &kv : groups) {
-        const scrs_job &j0 = jobs[kv.second.front()];
+    auto same_range = [](const EncKey &a, const EncKey &b) {
+        return a.engine == b.engine && a.k == b.k && a.m == b.m && a.L == b.L && a.cls == b.cls;
+    };
+    for (size_t g0 = 0; g0 < enc_keys_.size();) {
+        size_t g1 = g0 + 1;
+        while (g1 < enc_keys_.size() && same_range(enc_keys_[g0], enc_keys_[g1])) g1++;
+        const scrs_job &j0 = jobs[enc_keys_[g0].job];
         const scrs_profile &pr = j0.profile;
         scrs_status es = ok();
         CacheEntry *e = encoder_tables(pr, p, &es);
@@ -304,38 +316,53 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
         r.rows = static_cast<uint32_t>(m);
         r.groups = e->groups;
         r.tiles = static_cast<uint32_t>((L + tile - 1) / tile);
-        r.cw_count = static_cast<uint32_t>(kv.second.size());
+        r.cw_count = static_cast<uint32_t>(g1 - g0);
         r.ncw = std::min<uint32_t>(cw, r.cw_count);
-        const int cls = std::get<4>(kv.first);
+        const int cls = enc_keys_[g0].cls;
         r.flags = (cls == 2 ? kRangeVec16 | kRangeDst4 : cls == 1 ? kRangeDst4 : 0u) |
                   (engine_is_gf256(pr.engine) ? 0u : kRangeGf16);
         Plan::RangeRef ref;
         ref.src = p.ptrs.size();
-        for (uint64_t ji : kv.second)
-            for (uint64_t t = 0; t < k; t++) p.ptrs.push_back(reinterpret_cast<uint64_t>(jobs[ji].src + t * L));
+        for (size_t g = g0; g < g1; g++) {
+            const uint8_t *src = jobs[enc_keys_[g].job].src;
+            for (uint64_t t = 0; t < k; t++) p.ptrs.push_back(reinterpret_cast<uint64_t>(src + t * L));
+        }
         ref.dst = p.ptrs.size();
-        for (uint64_t ji : kv.second)
-            for (uint64_t t = 0; t < m; t++) p.ptrs.push_back(reinterpret_cast<uint64_t>(jobs[ji].dst + t * L));
+        for (size_t g = g0; g < g1; g++) {
+            uint8_t *dst = jobs[enc_keys_[g].job].dst;
+            for (uint64_t t = 0; t < m; t++) p.ptrs.push_back(reinterpret_cast<uint64_t>(dst + t * L));
+        }
         if (!add_range(r, ref)) return st(SCRS_ERR_CAPACITY, 0xffffffffull);
+        g0 = g1;
     }
-    for (uint64_t ji : decodes) {
+    for (uint64_t ji : decodes_) {
         const scrs_job &j = jobs[ji];
         const scrs_profile &pr = j.profile;
         const uint64_t k = pr.k, m = pr.m, n = k + m, L = pr.symbol_len;
         const bool gf16 = !engine_is_gf256(pr.engine);
-        // Terms in ascending index order: the recipe depends on the set only.
-        std::vector<std::pair<uint16_t, const uint8_t *>> rec(k);
-        for (uint64_t t = 0; t < k; t++) rec[t] = {j.indices[t], j.payloads[t]};
-        std::sort(rec.begin(), rec.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-        std::vector<uint64_t> key((n + 63) / 64 + 3, 0);
-        bool vec = L % 16 == 0 && aligned16(j.dst);
-        for (const auto &x : rec) {
-            key[x.first >> 6] |= 1ull << (x.first & 63);
-            vec = vec && aligned16(x.second);
+        // Terms in ascending index order (the recipe depends on the set
+        // only): mark the received indices, then read the bitmap in order.
+        // pos_ is written before it is read for every marked index.
+        std::vector<uint64_t> &key = key_;
+        key.assign((n + 63) / 64 + 3, 0);
+        if (pos_.size() < n) pos_.resize(n);
+        for (uint64_t t = 0; t < k; t++) {
+            const uint16_t x = j.indices[t];
+            key[x >> 6] |= 1ull << (x & 63);
+            pos_[x] = static_cast<uint32_t>(t);
         }
-        std::vector<uint16_t> missing;
+        order_.clear();
+        missing_.clear();
+        bool vec = L % 16 == 0 && aligned16(j.dst);
+        for (uint64_t w = 0; w < (n + 63) / 64; w++)
+            for (uint64_t bits = key[w]; bits; bits &= bits - 1) {
+                const uint16_t x = static_cast<uint16_t>(64 * w + __builtin_ctzll(bits));
+                order_.push_back(x);
+                vec = vec && aligned16(j.payloads[pos_[x]]);
+            }
         for (uint64_t i = 0; i < k; i++)
-            if (!((key[i >> 6] >> (i & 63)) & 1)) missing.push_back(static_cast<uint16_t>(i));
+            if (!((key[i >> 6] >> (i & 63)) & 1)) missing_.push_back(static_cast<uint16_t>(i));
+        const std::vector<uint16_t> &missing = missing_;
         const uint32_t r = static_cast<uint32_t>(missing.size());
         key[key.size() - 3] = static_cast<uint64_t>(pr.engine);
         key[key.size() - 2] = k;
@@ -376,10 +403,10 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
                 sv.coords = coords_[coord_slot(pr.engine)].as<uint16_t>();
                 Plan::SolveRef sr;
                 sr.idx = p.u16s.size();
-                for (const auto &x : rec) p.u16s.push_back(x.first);
+                for (uint16_t x : order_) p.u16s.push_back(x);
                 sr.slot = p.u16s.size();
                 uint16_t rank = 0;
-                for (const auto &x : rec) p.u16s.push_back(x.first < k ? kDataSlot : rank++);
+                for (uint16_t x : order_) p.u16s.push_back(x < k ? kDataSlot : rank++);
                 sr.missing = p.u16s.size();
                 for (uint16_t v : missing) p.u16s.push_back(v);
                 sr.work = p.work_u32;
@@ -393,12 +420,12 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
         }
         Plan::RangeRef ref;
         ref.src = p.ptrs.size();
-        for (const auto &x : rec) p.ptrs.push_back(reinterpret_cast<uint64_t>(x.second));
+        for (uint16_t x : order_) p.ptrs.push_back(reinterpret_cast<uint64_t>(j.payloads[pos_[x]]));
         ref.dst = p.ptrs.size();
         for (uint16_t v : missing) p.ptrs.push_back(reinterpret_cast<uint64_t>(j.dst + v * L));
         ref.copy = p.ptrs.size();
         ref.has_copy = true;
-        for (const auto &x : rec) p.ptrs.push_back(x.first < k ? reinterpret_cast<uint64_t>(j.dst + x.first * L) : 0);
+        for (uint16_t x : order_) p.ptrs.push_back(x < k ? reinterpret_cast<uint64_t>(j.dst + x * L) : 0);
         if (!add_range(br, ref)) return st(SCRS_ERR_CAPACITY, 0xffffffffull);
     }
     split_terms(p, units);
@@ -561,18 +588,19 @@ scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_
     // Autotuner signature: job-count and mean-length buckets, field mix and
     // whether any encode geometry repeats (codewords per unit matters).
     uint64_t lsum = 0, enc = 0, gf16 = 0;
-    std::map<std::tuple<int32_t, uint64_t, uint64_t, uint64_t>, uint32_t> rep;
-    uint32_t max_rep = 0;
+    sig_keys_.clear();
     for (uint64_t i = 0; i < count; i++) {
         const scrs_job &j = jobs[i];
         lsum += j.profile.symbol_len;
         gf16 += engine_is_gf256(j.profile.engine) ? 0 : 1;
         if (j.op == SCRS_JOB_ENCODE) {
             enc++;
-            max_rep = std::max(max_rep, ++rep[std::make_tuple(j.profile.engine, j.profile.k, j.profile.m,
-                                                             j.profile.symbol_len)]);
+            sig_keys_.emplace_back(j.profile.engine, j.profile.k, j.profile.m, j.profile.symbol_len);
         }
     }
+    std::sort(sig_keys_.begin(), sig_keys_.end());
+    const uint32_t max_rep =
+        std::adjacent_find(sig_keys_.begin(), sig_keys_.end()) != sig_keys_.end() ? 2u : (enc ? 1u : 0u);
     const std::vector<uint32_t> sig = {log2_floor(lsum / count), gf16 ? 1u : 0u,
                                        static_cast<uint32_t>(4 * enc / (count + 1)), max_rep > 1 ? 1u : 0u};
     uint32_t tile = cfg_.tile_bytes, cw = cfg_.cw_per_unit;

@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/scrs.h"
@@ -44,6 +45,18 @@ struct CacheEntry {
     bool recipe = false;
     std::tuple<int32_t, uint64_t, uint64_t> enc_key;
     std::vector<uint64_t> recipe_key;
+};
+
+// Recipe keys: received-index bitmap, then engine, k and m.
+struct RecipeKeyHash {
+    size_t operator()(const std::vector<uint64_t> &v) const {
+        uint64_t h = 0x9E3779B97F4A7C15ull ^ v.size();
+        for (uint64_t w : v) {
+            h ^= w + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+            h = (h ^ (h >> 31)) * 0xBF58476D1CE4E5B9ull;
+        }
+        return static_cast<size_t>(h ^ (h >> 29));
+    }
 };
 
 class BatchEngine {
@@ -99,12 +112,28 @@ class BatchEngine {
     hipEvent_t last_done_ = nullptr;  // the previous batch, on whatever stream
     // Resident caches, least recently used first.
     std::map<std::tuple<int32_t, uint64_t, uint64_t>, CacheEntry *> enc_cache_;
-    std::map<std::vector<uint64_t>, CacheEntry *> recipe_cache_;
+    std::unordered_map<std::vector<uint64_t>, CacheEntry *, RecipeKeyHash> recipe_cache_;
     std::list<CacheEntry *> lru_;
     std::vector<CacheEntry *> retired_;
     uint64_t cache_bytes_ = 0;
     std::deque<std::pair<uint64_t, hipEvent_t>> inflight_;
     uint64_t completed_ = 0;
+
+    // Planner scratch reused across jobs and batches (no per-job allocation).
+    struct EncKey {
+        int32_t engine;
+        int cls;
+        uint64_t k, m, L, job;
+        bool operator<(const EncKey &o) const {
+            return std::tie(engine, k, m, L, cls, job) < std::tie(o.engine, o.k, o.m, o.L, o.cls, o.job);
+        }
+    };
+    std::vector<EncKey> enc_keys_;
+    std::vector<uint64_t> decodes_, key_;
+    mutable std::vector<uint64_t> seen_;  // validate(), under mu_
+    std::vector<uint32_t> pos_;     // wire index -> position in the job's index list
+    std::vector<uint16_t> order_, missing_;
+    std::vector<std::tuple<int32_t, uint64_t, uint64_t, uint64_t>> sig_keys_;
 
     // Autotuner: batch signature -> (tile bytes, codewords per unit).
     std::map<std::vector<uint32_t>, std::pair<uint32_t, uint32_t>> tuned_;
