@@ -268,31 +268,67 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
             }
         }
         if (der) {
-            // D[p] = sum over pass bits beta clear in p of d_(b_lo+beta) * C[p | 2^beta],
-            // read from the group's coefficients in LDS.
-            store_lds(stage_lo(cur), stage_t(cur));
-            __syncthreads();
+            // D[p] = sum over pass bits beta clear in p of d_(b_lo+beta) * C[p | 2^beta].
+            // In a stage layout the stage's bits of a register's row are
+            // bits of the register index, so each stage's terms pair
+            // registers j and j | 2^bs with no branch and no LDS read.  The
+            // current stage is summed in place; each other stage is summed
+            // in its own layout and brought back through the LDS tile.
+            // Source-major: each coefficient c[i] is scaled by the table of
+            // every stage bit set in i and added to the row with that bit
+            // clear, so its nibble offsets serve all of its products (the
+            // tables of one stage are consecutive).
+            auto der_stage = [&](const uint32_t(&c)[R], uint32_t(&d)[R], uint32_t s_lo, uint32_t t) {
+#pragma unroll
+                for (int j = 0; j < R; j++) d[j] = 0;
+                const uint16_t *tab0 = der_t + (b_lo + s_lo) * kTab;
+#pragma unroll
+                for (int i = 1; i < R; i++) {
+#pragma unroll
+                    for (int bs = 0; bs < static_cast<int>(RB); bs++)
+                        if ((i >> bs) & 1)
+                            if (static_cast<uint32_t>(bs) < t) d[i ^ (1 << bs)] ^= gmul2(tab0 + bs * kTab, c[i]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            };
+            uint32_t acc[R];
+            der_stage(v, acc, stage_lo(cur), stage_t(cur));
+            __builtin_amdgcn_sched_barrier(0);
+            // A pass has at most two stages (g <= 8 bits, 4 bits per stage
+            // at R = 16, one stage below).  With two, the other stage's
+            // terms are summed in its own layout: the coefficients go
+            // through the tile into v in that layout, this stage's partial
+            // sums take their place in the tile, and the result continues
+            // in the other layout.
+            if (nst == 2) {
+                const uint32_t k = 1u - static_cast<uint32_t>(cur);
+                store_lds(stage_lo(cur), stage_t(cur));
+                __syncthreads();
+                load_lds(stage_lo(k), stage_t(k));
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < R; j++) data[rho<R>(wv, stage_lo(cur), stage_t(cur), j) * kTileDwords + lane] = acc[j];
+                der_stage(v, acc, stage_lo(k), stage_t(k));
+                __builtin_amdgcn_sched_barrier(0);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < R; j++) acc[j] ^= data[rho<R>(wv, stage_lo(k), stage_t(k), j) * kTileDwords + lane];
+                __syncthreads();  // the tile is free for the next layout change
+                cur = static_cast<int>(k);
+            }
 #pragma unroll
             for (int j = 0; j < R; j++) {
-                const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
-                uint32_t acc = 0;
-                // Unrolled over the at most 8 pass bits so the coefficient reads
-                // and table lookups of all terms are in flight together.
-#pragma unroll
-                for (uint32_t beta = 0; beta < 8; beta++)
-                    if (beta < g && !((p >> beta) & 1u))
-                        acc ^= gmul2(der_t + (b_lo + beta) * kTab, data[(p | (1u << beta)) * kTileDwords + lane]);
                 if (a.aux_base && (A4 || valid)) {
+                    const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
                     const uint32_t r = base | (p << b_lo);
                     const uint8_t *row = a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L;
                     if constexpr (A4)
-                        acc ^= ld_col4(row, a.node_tabs, static_cast<uint32_t>(cb), L);
+                        acc[j] ^= ld_col4(row, a.node_tabs, static_cast<uint32_t>(cb), L);
                     else
-                        acc ^= ld_col(row, cb, L);
+                        acc[j] ^= ld_col(row, cb, L);
                 }
-                v[j] = acc;
+                v[j] = acc[j];
             }
-            __syncthreads();
         }
         if (fwd) {
             for (int k = static_cast<int>(nst) - 1; k >= 0; k--) {
