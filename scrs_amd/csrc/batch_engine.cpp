@@ -483,12 +483,12 @@ hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
         hipGraphExec_t exec = nullptr;
         hipError_t err = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal);
         if (err != hipSuccess) return err;
-        // The queue counter is cleared by a kernel node: no memset nodes.
-        hipError_t e1 = launch_batch_reset(slot.counter.as<uint32_t>(), capture_);
-        hipError_t e2 = launch_batch_solve(hdr, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, capture_);
+        // The queue counter is cleared by the solve kernel: no memset nodes.
+        hipError_t e2 = launch_batch_solve(hdr, gf8_.as<uint8_t>(), gf16_log_exp_, slot.counter.as<uint32_t>(),
+                                           kSolveGrid, capture_);
         hipError_t e3 = launch_batch_rows(hdr, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), capture_);
         err = hipStreamEndCapture(capture_, &graph);
-        if (err == hipSuccess) err = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3);
+        if (err == hipSuccess) err = e2 != hipSuccess ? e2 : e3;
         if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (graph) (void)hipGraphDestroy(graph);
         if (err != hipSuccess) return err;
@@ -563,8 +563,8 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
         err = run_graph(slot, tile, s);
     } else {
         const BatchHeader *dh = slot.dev.as<BatchHeader>();
-        if ((err = launch_batch_reset(slot.counter.as<uint32_t>(), s)) == hipSuccess &&
-            (err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, s)) == hipSuccess)
+        if ((err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, slot.counter.as<uint32_t>(), kSolveGrid,
+                                      s)) == hipSuccess)
             err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), s);
     }
     if (err != hipSuccess) return err;

@@ -143,7 +143,9 @@ __device__ void tables_job16(const BatchSolve &j, const FieldGf65536 &f) {
 }
 
 __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr, const uint8_t *gf8,
-                                                         const uint16_t *gf16_log_exp) {
+                                                         const uint16_t *gf16_log_exp, uint32_t *counter) {
+    // The multiply-XOR launch that follows takes units from this counter.
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
     __shared__ uint8_t exp[512];
     __shared__ uint8_t log[256];
     for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) exp[i] = gf8[i];
@@ -176,10 +178,6 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
     }
 }
 
-__global__ void batch_reset_kernel(uint32_t *counter) {
-    if (threadIdx.x == 0) *counter = 0;
-}
-
 // ---- multiply-XOR ----------------------------------------------------------
 
 template <bool VEC, bool DEC, int RW, int TPU>
@@ -209,11 +207,14 @@ __device__ __forceinline__ void unit8(const BatchRange &R, uint32_t cw0, uint32_
                 __syncthreads();
             }
             if (!active) continue;
+            // One term ahead: the next source loads while this one folds.
+            uint32_t wn[4];
+            load16<VEC>(load_uniform(R.src + c * R.T + t0), p, R.L, wn);
 #pragma unroll 1
             for (uint32_t t = 0; t < tc; t++) {
                 const uint32_t gt = c * R.T + t0 + t;
-                uint32_t w[4];
-                load16<VEC>(load_uniform(R.src + gt), p, R.L, w);
+                const uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
+                if (t + 1 < tc) load16<VEC>(load_uniform(R.src + gt + 1), p, R.L, wn);
                 if (DEC && do_copy) {
                     uint8_t *cp = load_uniform(R.copy + gt);
                     if (cp) store16<VEC>(cp, p, R.L, w);
@@ -267,11 +268,13 @@ __device__ __forceinline__ void unit16(const BatchRange &R, uint32_t cw0, uint32
                 __syncthreads();
             }
             if (!active) continue;
+            uint32_t wn[4];
+            load16<VEC>(load_uniform(R.src + c * R.T + t0), p, R.L, wn);
 #pragma unroll 1
             for (uint32_t t = 0; t < tc; t++) {
                 const uint32_t gt = c * R.T + t0 + t;
-                uint32_t w[4];
-                load16<VEC>(load_uniform(R.src + gt), p, R.L, w);
+                const uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
+                if (t + 1 < tc) load16<VEC>(load_uniform(R.src + gt + 1), p, R.L, wn);
                 if (DEC && do_copy) {
                     uint8_t *cp = load_uniform(R.copy + gt);
                     if (cp) store16<VEC>(cp, p, R.L, w);
@@ -380,14 +383,9 @@ __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr,
 
 }  // namespace
 
-hipError_t launch_batch_reset(uint32_t *counter, hipStream_t s) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, counter);
-    return hipGetLastError();
-}
-
 hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, const uint16_t *gf16_log_exp,
-                              uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables, gf16_log_exp);
+                              uint32_t *counter, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables, gf16_log_exp, counter);
     return hipGetLastError();
 }
 
