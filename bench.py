@@ -276,7 +276,7 @@ def serve_workload(args) -> dict:
     plain_half = run(plain, False, load)
     batched_half = run(batched, True, load)
     best = None
-    for mult in (2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96):
+    for mult in (2, 4, 6, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96):
         rate = load * mult
         if rate > 1.2 * batched_sat["req_per_s"]:
             break

@@ -23,8 +23,14 @@ namespace {
 constexpr uint32_t kSolveGrid = 256;
 constexpr uint16_t kDataSlot = 0xFFFF;
 constexpr uint64_t kTuneMinJobs = 32;
-constexpr uint64_t kFillUnits = 512;  // two units per CU
-constexpr uint32_t kMinSegTerms = 8;   // shortest term segment of a split range
+#ifndef SCRS_FILL_UNITS
+#define SCRS_FILL_UNITS 512
+#endif
+#ifndef SCRS_MIN_SEG_TERMS
+#define SCRS_MIN_SEG_TERMS 8
+#endif
+constexpr uint64_t kFillUnits = SCRS_FILL_UNITS;     // units a batch should offer the device
+constexpr uint32_t kMinSegTerms = SCRS_MIN_SEG_TERMS;  // shortest term segment of a split range
 
 uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t{15}; }
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
