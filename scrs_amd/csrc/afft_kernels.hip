@@ -66,7 +66,8 @@ __device__ __forceinline__ void st_col(uint8_t *p, uint64_t cb, uint64_t L, uint
         if (cb + b < L) p[cb + b] = static_cast<uint8_t>(w >> (8 * b));
 }
 
-// A4 (every row address and L a multiple of 4, L < 2^32): dword columns
+// A4 (every row address and L a multiple of 4, L < 2^31, so a column offset
+// past L still fits 32 bits): dword columns
 // through buffer instructions.  A null row or a column at or past L is a
 // zero load and a dropped store by the buffer's range, with no branch; the
 // branchy byte paths above are kept for unaligned rows.
@@ -498,7 +499,7 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     // multiple of 4 (row tables: as declared by the caller).
     auto al4 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; };
     const bool tables = a.in_rows || a.copy_rows || a.out_rows;
-    const bool a4 = a.L % 4 == 0 && a.L <= 0xffffffffull && al4(a.in_base) && al4(a.out_base) && al4(a.aux_base) &&
+    const bool a4 = a.L % 4 == 0 && a.L <= 0x7fffffffull && al4(a.in_base) && al4(a.out_base) && al4(a.aux_base) &&
                     (a.in_cw_stride | a.out_cw_stride | a.aux_cw_stride) % 4 == 0 && (!tables || a.rows_aligned4);
     if (a4) {
         switch (a.g) {
