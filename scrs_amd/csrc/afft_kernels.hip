@@ -66,12 +66,11 @@ __device__ __forceinline__ void st_col(uint8_t *p, uint64_t cb, uint64_t L, uint
         if (cb + b < L) p[cb + b] = static_cast<uint8_t>(w >> (8 * b));
 }
 
-// A4 (every row address and L a multiple of 4, L < 2^31, so a column offset
-// past L still fits 32 bits): dword columns
-// through buffer instructions.  A null row or a column at or past L is a
-// zero load and a dropped store by the buffer's range, with no branch; the
-// branchy byte paths above are kept for unaligned rows.
-typedef uint32_t v1u32 __attribute__((ext_vector_type(1)));
+// A4 (every row address and L a multiple of 4, L < 2^31 so that a column
+// offset past L still fits 32 bits): dword columns through buffer
+// instructions.  A null row or a column at or past L is a zero load and a
+// dropped store by the buffer's range, with no branch; the branchy byte
+// paths above are kept for unaligned rows.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const uint8_t *row, const void *dummy, uint64_t L) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(row ? row : static_cast<const uint8_t *>(dummy)),
                                              static_cast<short>(0), row ? static_cast<uint32_t>(L) : 0u,
