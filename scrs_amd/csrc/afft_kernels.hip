@@ -34,12 +34,22 @@ __device__ __forceinline__ T *load_uniform(T *const *p) {
     return reinterpret_cast<T *>(*(const_u64_ptr)(p));
 }
 
+// Byte address of an LDS pointer, and a u16 read at a byte address.
+__device__ __forceinline__ uint32_t lds_byte(const uint16_t *p) {
+    return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((const __attribute__((address_space(3))) uint16_t *)(p)));
+}
+__device__ __forceinline__ uint32_t lds_u16(uint32_t a) {
+    return *(const __attribute__((address_space(3))) uint16_t *)(static_cast<uintptr_t>(a));
+}
+
+// Every constant table starts on a multiple of 128 bytes of the 128-byte
+// aligned LDS block, so an entry's address is the table address OR the
+// entry offset: one AND-OR per lookup after the shift.
 __device__ __forceinline__ uint32_t gmul(const uint16_t *t, uint32_t x) {
-    // Tables are 128-byte aligned in LDS, so an entry address is the table
-    // base OR the entry offset.
-    t = static_cast<const uint16_t *>(__builtin_assume_aligned(t, 128));
-    return static_cast<uint32_t>(t[x & 15u]) ^ t[16u | ((x >> 4) & 15u)] ^ t[32u | ((x >> 8) & 15u)] ^
-           t[48u | ((x >> 12) & 15u)];
+    const uint32_t a = lds_byte(t);
+    __builtin_assume((a & 127u) == 0);
+    return lds_u16(a | ((x << 1) & 0x1Eu)) ^ lds_u16((a + 32u) | ((x >> 3) & 0x1Eu)) ^
+           lds_u16((a + 64u) | ((x >> 7) & 0x1Eu)) ^ lds_u16((a + 96u) | ((x >> 11) & 0x1Eu));
 }
 
 // Two packed elements times the same constant.
@@ -98,7 +108,7 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 template <int R, bool A4>
 __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_t tpb) {
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __align__(128) uint32_t smem[];  // tables at multiples of 128 bytes
     const uint32_t g = a.g, G = 1u << g, b_lo = a.b_lo;
     const uint32_t gidx = blockIdx.y;
     const uint32_t base = (gidx & ((1u << b_lo) - 1u)) | ((gidx >> b_lo) << (b_lo + g));
