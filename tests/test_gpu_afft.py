@@ -205,6 +205,39 @@ def test_decode_batch_dev_and_ptrs(torch_cuda):
     assert torch.equal(out2, data)
 
 
+@pytest.mark.parametrize("shift", [2, 4])
+def test_batch_dev_misaligned_buffers(torch_cuda, shift):
+    """symbol_len a multiple of 4 with buffers 2 bytes off (byte path) and 4
+    bytes off (dword buffer path): the same bytes as the aligned call, and
+    the repairs against the oracle."""
+    torch = torch_cuda
+    k, m, L, count = 24, 12, 64, 3
+    p = Profile.resolve(AFFT, k, m, L)
+    base = torch.empty(count * k * L + 16, dtype=torch.uint8, device="cuda")
+    fill_random_dev(base, base.numel(), 0xA11 + shift)
+    data = base[shift:shift + count * k * L]
+    rep_buf = torch.full((count * m * L + 16,), 0xAB, dtype=torch.uint8, device="cuda")
+    rep = rep_buf[shift:shift + count * m * L]
+    BatchEncoder(p).encode_batch_dev(data, rep, count)
+    aligned = data.clone()
+    rep_al = torch.empty(count * m * L, dtype=torch.uint8, device="cuda")
+    BatchEncoder(p).encode_batch_dev(aligned, rep_al, count)
+    torch.cuda.synchronize()
+    assert torch.equal(rep, rep_al)
+    d0 = aligned[:k * L].cpu().numpy()
+    want = oracle.afft_encode(k, m, L, d0)
+    assert np.array_equal(rep_al[:m * L].cpu().numpy(), want)
+    rng = np.random.default_rng(shift)
+    idx = np.zeros((count, k), np.uint16)
+    for c in range(count):
+        idx[c] = np.sort(rng.choice(k + m, size=k, replace=False))
+    out_buf = torch.zeros(count * k * L + 16, dtype=torch.uint8, device="cuda")
+    out = out_buf[shift:shift + count * k * L]
+    BatchDecoder(p).decode_batch_dev(idx, data, rep, out, count)
+    torch.cuda.synchronize()
+    assert torch.equal(out, aligned)
+
+
 def test_full_domain_32768_32768_2048(torch_cuda):
     """k = m = 32768, symbol_len = 2048 (128 MiB codeword): fused encode,
     decode from a seeded random half of the 65536 indices, round trip, and
