@@ -29,6 +29,8 @@ SEED_DATA = 0x5C25_0002
 SEED_ERASE = 0x5C25_E7A5
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
+# LDS read bandwidth: 256 CUs x 232 B/clk (ds_read_b128, measured) x 2.4 GHz.
+LDS_PEAK_TBS = 256 * 232 * 2.4e9 / 1e12
 
 
 def erasure_indices(count: int, k: int, m: int, max_r: int, seed: int) -> np.ndarray:
@@ -150,6 +152,18 @@ def gf16_workload(args) -> dict:
     if not torch.equal(out, data):
         raise SystemExit("decode output differs from the original data")
     enc_bytes, dec_bytes = count * (k + m) * sl, count * 2 * k * sl
+    lds_roof = None
+    if args.workload == "tower":
+        # The feed reads, per element and term, four nibble-table entries of
+        # each 16-row group: 2 bytes per row of the group, 8 bytes for groups
+        # of at most 4 rows and 16 for at most 8 (the staged entry sizes).
+        group_rows = [min(16, m - g) for g in range(0, m, 16)]
+        entry = sum(8 if r <= 4 else (16 if r <= 8 else 32) for r in group_rows)
+        lds_bytes = count * k * (sl // 2) * 4 * entry
+        lds_tbs = lds_bytes / (enc_ms / 1e3) / 1e12
+        lds_roof = {"kernel": "rows_terms16_kernel, feed", "achieved": round(lds_tbs, 2),
+                    "peak": LDS_PEAK_TBS, "unit": "TB/s", "frac": round(lds_tbs / LDS_PEAK_TBS, 4),
+                    "lds_bytes_per_launch": lds_bytes}
     value = (enc_bytes + dec_bytes) / (wall / args.steps) / 1e9
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
     dec_gbs = dec_bytes / (dec_ms / 1e3) / 1e9
@@ -167,6 +181,7 @@ def gf16_workload(args) -> dict:
                      # GF(2^16) work is bound by LDS table lookups, not HBM: the
                      # fraction is of the HBM peak for comparison only.
                      "limiter": "lds"},
+        "lds_roofline": lds_roof,
         "cpu_baseline": None,
     }
 
