@@ -498,6 +498,7 @@ def main() -> None:
     host = host_slice_rates(enc, dec, data, rep, idx) if not args.no_host_slice else None
 
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
+    enc_lds_bytes = count * K * L * 32  # two 16-byte entries per source byte, M = 16 rows
     dec_gbs = dec_bytes / (dec_ms / 1e3) / 1e9
     traffic = traffic_enc = None
     tf = ROOT / "profiles" / "traffic.json"
@@ -549,6 +550,11 @@ def main() -> None:
             "unit": "GB/s",
             "frac": round(enc_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic_enc,
+            # Each source byte reads two 16-byte table entries (all 16 rows):
+            # the encode is bound by LDS reads as much as by HBM.
+            "lds_achieved_tbs": round(enc_lds_bytes / (enc_ms / 1e3) / 1e12, 2),
+            "lds_peak_tbs": round(LDS_PEAK_TBS, 2),
+            "lds_frac": round(enc_lds_bytes / (enc_ms / 1e3) / 1e12 / LDS_PEAK_TBS, 4),
         },
         "cpu_baseline": cpu,
         "bitmatrix_kernel": bitmatrix,
