@@ -125,12 +125,17 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
                 __builtin_amdgcn_sched_barrier(0);
             }
         } else {
+            // One term ahead: the next source loads while this one folds.
+            auto src_of = [&](uint64_t gt) -> const uint8_t * {
+                return DEC ? load_uniform(a.src_ptrs + c * a.T + gt) : src_cw + gt * a.L;
+            };
+            uint32_t wn[4];
+            load16<VEC>(src_of(t0), p, a.L, wn);
 #pragma unroll 1
             for (uint32_t t = 0; t < tc; t++) {
                 const uint64_t gt = t0 + t;
-                const uint8_t *src = DEC ? load_uniform(a.src_ptrs + c * a.T + gt) : src_cw + gt * a.L;
-                uint32_t w[4];
-                load16<VEC>(src, p, a.L, w);
+                const uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
+                if (t + 1 < tc) load16<VEC>(src_of(gt + 1), p, a.L, wn);
                 if (DEC && do_copy) {
                     uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + gt);
                     if (cp) store16<VEC>(cp, p, a.L, w);
