@@ -67,6 +67,17 @@ const char *scrs_status_name(int32_t code);
  * unchanged. */
 uint64_t scrs_device_alloc_count(void);
 
+/* Term split of small codec launches (no reference counterpart).  When
+ * enabled, a codec launch too small to fill the device (a lone codeword,
+ * say) splits each codeword's terms across workgroups that XOR their
+ * partial rows into destination rows cleared first; it applies to launches
+ * whose destination rows and symbol_len are multiples of 4 bytes and that
+ * overwrite their destinations.  The bytes are the same either way.  Off by
+ * default: the codec entry points then run the plain per-codeword path, one
+ * workgroup per row group and column tile folding every term.
+ * Process-wide; returns the previous setting. */
+int32_t scrs_set_term_split(int32_t enable);
+
 /* ---- Profile selection ------------------------------------------------ */
 
 /* Engine::field (codec.rs:42-50).  Returns -1 for an unknown engine. */

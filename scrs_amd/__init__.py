@@ -30,5 +30,6 @@ from .codec import (  # noqa: F401
     fill_random_dev,
     now_ns,
     device_alloc_count,
+    set_term_split,
     recommended_gf16_engine,
 )

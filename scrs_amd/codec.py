@@ -717,6 +717,13 @@ def device_alloc_count() -> int:
     return int(_lib.load().scrs_device_alloc_count())
 
 
+def set_term_split(enable: bool) -> bool:
+    """Let codec launches too small to fill the device split each codeword's
+    terms across workgroups (process-wide, off by default).  Returns the
+    previous setting."""
+    return bool(_lib.load().scrs_set_term_split(1 if enable else 0))
+
+
 def now_ns() -> int:
     return int(_lib.load().scrs_now_ns())
 

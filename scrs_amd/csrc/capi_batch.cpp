@@ -251,6 +251,7 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
     a.groups = enc->groups;
     a.L = L;
     a.count = count;
+    a.dst_aligned4 = aligned4(repairs_dev) && L % 4 == 0;
     hipError_t err;
     if (enc->kernel == SCRS_KERNEL_BITMATRIX && aligned4(data_dev) && aligned4(repairs_dev)) {
         a.tables = enc->d_bm_tables.as<uint32_t>();

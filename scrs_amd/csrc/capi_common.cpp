@@ -1,8 +1,21 @@
 // capi_common.cpp - status names and the additive-FFT hooks.
+#include <atomic>
+
 #include "handles.hpp"
 #include "status.hpp"
 
 extern "C" uint64_t scrs_device_alloc_count(void) { return scrs::alloc_counter().load(std::memory_order_relaxed); }
+
+namespace scrs {
+namespace {
+std::atomic<int32_t> g_term_split{0};
+}  // namespace
+bool term_split_enabled() { return g_term_split.load(std::memory_order_relaxed) != 0; }
+}  // namespace scrs
+
+extern "C" int32_t scrs_set_term_split(int32_t enable) {
+    return scrs::g_term_split.exchange(enable ? 1 : 0, std::memory_order_relaxed);
+}
 
 extern "C" const char *scrs_status_name(int32_t code) {
     switch (code) {

@@ -552,6 +552,7 @@ extern "C" scrs_status scrs_incr_encode_batch_dev(const scrs_incr_encoder *enc, 
     a.groups = enc->groups;
     a.L = L;
     a.count = count;
+    a.dst_aligned4 = (reinterpret_cast<uintptr_t>(repairs_dev) & 3u) == 0 && L % 4 == 0;
     hipError_t err = launch_rows_terms_field(enc->gf16, a, vec16, static_cast<hipStream_t>(stream));
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }

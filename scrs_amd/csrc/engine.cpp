@@ -179,6 +179,8 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         ra.groups = groups;
         ra.L = L;
         ra.count = cn;
+        // Missing rows are out + index * L.
+        ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
         err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);
         if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     }
@@ -230,6 +232,7 @@ scrs_status DecodeEngine::enqueue_recipe(const uint16_t *indices, const uint32_t
     ra.groups = groups;
     ra.L = L;
     ra.count = 1;
+    ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
     err = launch_rows_terms_field(gf16, ra, vec16, s);
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }

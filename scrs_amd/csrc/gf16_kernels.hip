@@ -34,7 +34,7 @@ using namespace rowsdev;
 // over chunks of kTermChunk terms.
 template <bool VEC, bool DEC, int RW, int FULL>
 __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here, uint64_t p,
-                       bool do_copy) {
+                       bool do_copy, uint32_t tb, uint32_t te) {
     constexpr int EB = 2 * RW;
     constexpr int D = EB / 4;
     uint32_t acc[8][D];
@@ -46,8 +46,8 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
     const uint8_t *tab_cw = reinterpret_cast<const uint8_t *>(a.tables + c * a.table_cw_stride);
     const uint8_t *src_cw = DEC ? nullptr : a.src_base + c * a.src_cw_stride;
     constexpr uint32_t kChunk = FULL ? FULL : kTermChunk;
-    for (uint32_t t0 = 0; t0 < a.T; t0 += kChunk) {
-        const uint32_t tc = min(kChunk, a.T - t0);
+    for (uint32_t t0 = tb; t0 < te; t0 += kChunk) {
+        const uint32_t tc = min(kChunk, te - t0);
         __syncthreads();
         // Stage: entry (q, n) of term t keeps its first EB bytes (rows 0..RW-1).
         for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += kThreads) {
@@ -145,6 +145,10 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
 #pragma unroll
         for (int j = 0; j < 4; j++)
             o[j] = ((acc[2 * j][d] >> sh) & 0xffffu) | (((acc[2 * j + 1][d] >> sh) & 0xffffu) << 16);
+        if (a.segs > 1) {  // a term segment: XOR into the cleared row
+            xor16_atomic(dst, p, a.L, o);
+            continue;
+        }
         if (a.accumulate) {
             uint32_t prev[4];
             load16<VEC>(dst, p, a.L, prev);
@@ -156,9 +160,9 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
 }
 
 template <bool VEC>
-__device__ void copy_only16(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
+__device__ void copy_only16(const RowsTermsArgs &a, uint64_t c, uint64_t p, uint32_t tb, uint32_t te) {
     if (p >= a.L) return;
-    for (uint32_t t = 0; t < a.T; t++) {
+    for (uint32_t t = tb; t < te; t++) {
         uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + t);
         if (!cp) continue;
         uint32_t w[4];
@@ -175,8 +179,11 @@ __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a,
     const uint32_t unit = xcd + kXcds * (slot / tiles);
     if (unit >= units) return;
     const uint32_t tile = slot % tiles;
-    const uint64_t c = unit / a.groups_launched;
-    const int rg = static_cast<int>(unit % a.groups_launched);
+    // Unit = (codeword, row group, term segment).
+    const uint32_t seg = unit % a.segs, cg = unit / a.segs;
+    const uint64_t c = cg / a.groups_launched;
+    const int rg = static_cast<int>(cg % a.groups_launched);
+    const uint32_t tb = seg * a.seg_terms, te = min(a.T, tb + a.seg_terms);
     const uint32_t R = DEC ? load_uniform(a.rows + c) : a.rmax;
     const uint32_t base = static_cast<uint32_t>(rg) * kRowsPerPass;
     const int rows_here = static_cast<int>(R > base ? min(static_cast<uint32_t>(kRowsPerPass), R - base) : 0u);
@@ -184,13 +191,13 @@ __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a,
     if (rows_here == 0 && !do_copy) return;
     const uint64_t p = static_cast<uint64_t>(tile) * kTile + threadIdx.x * 16;
     if (DEC && rows_here == 0)
-        copy_only16<VEC>(a, c, p);
+        copy_only16<VEC>(a, c, p, tb, te);
     else if (rows_here <= 4)
-        pass16<VEC, DEC, 4, FULL>(a, lds, c, rg, rows_here, p, do_copy);
+        pass16<VEC, DEC, 4, FULL>(a, lds, c, rg, rows_here, p, do_copy, tb, te);
     else if (rows_here <= 8)
-        pass16<VEC, DEC, 8, FULL>(a, lds, c, rg, rows_here, p, do_copy);
+        pass16<VEC, DEC, 8, FULL>(a, lds, c, rg, rows_here, p, do_copy, tb, te);
     else
-        pass16<VEC, DEC, 16, FULL>(a, lds, c, rg, rows_here, p, do_copy);
+        pass16<VEC, DEC, 16, FULL>(a, lds, c, rg, rows_here, p, do_copy, tb, te);
 }
 
 }  // namespace
@@ -204,7 +211,9 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
     const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
     if (groups > a.groups + (a.rmax == 0)) return hipErrorInvalidValue;
     a.groups_launched = groups;
-    const uint64_t units = a.count * groups;
+    // A launch too small to fill the device splits every codeword's terms.
+    const bool split = plan_term_split(a, a.count * groups * tiles);
+    const uint64_t units = a.count * groups * a.segs;
     const uint64_t blocks = (units + kXcds - 1) / kXcds * kXcds * tiles;
     if (blocks > 0x7fffffffull || units > 0xffffffffull) return hipErrorInvalidValue;
     const bool dec = a.src_ptrs != nullptr;
@@ -213,8 +222,13 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
     if (dec && a.accumulate) return hipErrorInvalidValue;
     const dim3 grid(static_cast<uint32_t>(blocks));
     const uint32_t t32 = static_cast<uint32_t>(tiles), u32 = static_cast<uint32_t>(units);
-    // Term counts that are multiples of 16 or 8 take the unrolled kernels.
-    const int full = !vec16 ? 0 : (a.T % 16 == 0 ? 16 : (a.T % 8 == 0 ? 8 : 0));
+    if (split) {
+        hipError_t err = launch_zero_dst_rows(a, s);
+        if (err != hipSuccess) return err;
+    }
+    // Term counts that are multiples of 16 or 8 take the unrolled kernels
+    // (whole codewords only: a split launch runs the rolled loop).
+    const int full = (!vec16 || split) ? 0 : (a.T % 16 == 0 ? 16 : (a.T % 8 == 0 ? 8 : 0));
     if (full == 16 && dec)
         hipLaunchKernelGGL((rows_terms16_kernel<true, true, 16>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (full == 16)

@@ -14,6 +14,8 @@
 // 4-byte entries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "rows_device.hpp"
 
@@ -42,7 +44,7 @@ using namespace rowsdev;
 // destinations are strided rows of back-to-back codewords (encode).
 template <bool VEC, bool DEC, bool FULL, int RW>
 __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here,
-                                uint64_t p, bool do_copy) {
+                                uint64_t p, bool do_copy, uint32_t tb, uint32_t te) {
     constexpr int D = RW / 4;
     uint32_t acc[16][D];
 #pragma unroll
@@ -53,8 +55,8 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
     const bool active = p < a.L;
     const uint32_t *tab_cw = a.tables + c * a.table_cw_stride;
     const uint8_t *src_cw = DEC ? nullptr : a.src_base + c * a.src_cw_stride;
-    for (uint32_t t0 = 0; t0 < a.T; t0 += kTermChunk) {
-        const uint32_t tc = min(static_cast<uint32_t>(kTermChunk), a.T - t0);
+    for (uint32_t t0 = tb; t0 < te; t0 += kTermChunk) {
+        const uint32_t tc = min(static_cast<uint32_t>(kTermChunk), te - t0);
         __syncthreads();
         // Stage this chunk's tables: entry n of term t keeps its first RW bytes.
         for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += kThreads) {
@@ -158,6 +160,10 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
             o[j] = ((acc[4 * j + 0][d] >> s) & 0xffu) | (((acc[4 * j + 1][d] >> s) & 0xffu) << 8) |
                    (((acc[4 * j + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * j + 3][d] >> s) & 0xffu) << 24);
         }
+        if (a.segs > 1) {  // a term segment: XOR into the cleared row
+            xor16_atomic(dst, p, a.L, o);
+            continue;
+        }
         if (!DEC && a.accumulate) {
             uint32_t prev[4];
             load16<VEC>(dst, p, a.L, prev);
@@ -169,9 +175,9 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
 }
 
 template <bool VEC>
-__device__ void copy_only_pass(const RowsTermsArgs &a, uint64_t c, uint64_t p) {
+__device__ void copy_only_pass(const RowsTermsArgs &a, uint64_t c, uint64_t p, uint32_t tb, uint32_t te) {
     if (p >= a.L) return;
-    for (uint32_t t = 0; t < a.T; t++) {
+    for (uint32_t t = tb; t < te; t++) {
         uint8_t *cp = load_uniform(a.copy_ptrs + c * a.T + t);
         if (!cp) continue;
         uint32_t w[4];
@@ -193,8 +199,11 @@ __global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs 
     const uint32_t unit = xcd + kXcds * (slot / tiles);
     if (unit >= units) return;
     const uint32_t tile = slot % tiles;
-    const uint64_t c = unit / a.groups_launched;
-    const int rg = static_cast<int>(unit % a.groups_launched);
+    // Unit = (codeword, row group, term segment).
+    const uint32_t seg = unit % a.segs, cg = unit / a.segs;
+    const uint64_t c = cg / a.groups_launched;
+    const int rg = static_cast<int>(cg % a.groups_launched);
+    const uint32_t tb = seg * a.seg_terms, te = min(a.T, tb + a.seg_terms);
     const uint32_t R = DEC ? load_uniform(a.rows + c) : a.rmax;
     const uint32_t base = static_cast<uint32_t>(rg) * kRowsPerPass;
     const int rows_here = static_cast<int>(R > base ? min(static_cast<uint32_t>(kRowsPerPass), R - base) : 0u);
@@ -202,14 +211,26 @@ __global__ __launch_bounds__(kThreads) void rows_terms_lds_kernel(RowsTermsArgs 
     if (rows_here == 0 && !do_copy) return;
     const uint64_t p = static_cast<uint64_t>(tile) * kTile + threadIdx.x * kBytesPerThread;
     if (DEC && rows_here == 0) {
-        copy_only_pass<VEC>(a, c, p);
+        copy_only_pass<VEC>(a, c, p, tb, te);
     } else if (rows_here <= 4) {
-        rows_terms_pass<VEC, DEC, FULL, 4>(a, lds, c, rg, rows_here, p, do_copy);
+        rows_terms_pass<VEC, DEC, FULL, 4>(a, lds, c, rg, rows_here, p, do_copy, tb, te);
     } else if (rows_here <= 8) {
-        rows_terms_pass<VEC, DEC, FULL, 8>(a, lds, c, rg, rows_here, p, do_copy);
+        rows_terms_pass<VEC, DEC, FULL, 8>(a, lds, c, rg, rows_here, p, do_copy, tb, te);
     } else {
-        rows_terms_pass<VEC, DEC, FULL, 16>(a, lds, c, rg, rows_here, p, do_copy);
+        rows_terms_pass<VEC, DEC, FULL, 16>(a, lds, c, rg, rows_here, p, do_copy, tb, te);
     }
+}
+
+// Destination rows of a term-split launch start from zero: one workgroup
+// per (row, codeword); rows are dword-aligned (the split requires it).
+__global__ __launch_bounds__(256) void zero_dst_rows_kernel(RowsTermsArgs a) {
+    const uint64_t c = blockIdx.y;
+    const uint32_t r = blockIdx.x;
+    const bool dec = a.dst_ptrs != nullptr;
+    if (dec && r >= load_uniform(a.rows + c)) return;
+    uint8_t *row = dec ? load_uniform(a.dst_ptrs + c * a.rmax + r)
+                       : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(r) * a.L;
+    for (uint64_t i = threadIdx.x; i < a.L / 4; i += blockDim.x) *gbl_mut<uint32_t>(row + 4 * i) = 0;
 }
 
 __global__ void fill_splitmix_kernel(uint8_t *dst, uint64_t len, uint64_t seed, uint64_t word0) {
@@ -230,6 +251,32 @@ __global__ void fill_splitmix_kernel(uint8_t *dst, uint64_t len, uint64_t seed, 
 
 }  // namespace
 
+bool plan_term_split(RowsTermsArgs &a, uint64_t active_blocks) {
+    a.segs = 1;
+    a.seg_terms = a.T;
+    if (!term_split_enabled() || a.accumulate || !a.dst_aligned4 || a.L % 4 || a.rmax == 0 ||
+        a.T < 2 * kSplitMinTerms ||
+        active_blocks >= kSplitFillBlocks || a.count > 65535)
+        return false;
+    const uint64_t want =
+        std::min<uint64_t>(a.T / kSplitMinTerms, (kSplitFillBlocks + active_blocks - 1) / active_blocks);
+    if (want < 2) return false;
+    a.seg_terms = static_cast<uint32_t>((a.T + want - 1) / want);
+    a.segs = (a.T + a.seg_terms - 1) / a.seg_terms;
+    if (a.segs < 2) {
+        a.segs = 1;
+        a.seg_terms = a.T;
+        return false;
+    }
+    return true;
+}
+
+hipError_t launch_zero_dst_rows(const RowsTermsArgs &a, hipStream_t s) {
+    if (a.count == 0 || a.rmax == 0 || a.count > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zero_dst_rows_kernel, dim3(a.rmax, static_cast<uint32_t>(a.count)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
     // The 16-byte paths address a row with a 32-bit column offset.
     if (args.L > 0xffffffffull) vec16 = false;
@@ -239,7 +286,9 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStrea
     const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
     if (groups > a.groups + (a.rmax == 0)) return hipErrorInvalidValue;
     a.groups_launched = groups;
-    const uint64_t units = a.count * groups;
+    // A launch too small to fill the device splits every codeword's terms.
+    const bool split = plan_term_split(a, a.count * groups * tiles);
+    const uint64_t units = a.count * groups * a.segs;
     const uint64_t blocks = (units + kXcds - 1) / kXcds * kXcds * tiles;
     if (blocks > 0x7fffffffull || units > 0xffffffffull) return hipErrorInvalidValue;
     const dim3 grid(static_cast<uint32_t>(blocks));
@@ -247,7 +296,11 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStrea
     const bool dec = a.src_ptrs != nullptr;
     if (dec != (a.dst_ptrs != nullptr) || dec != (a.copy_ptrs != nullptr) || dec != (a.rows != nullptr))
         return hipErrorInvalidValue;
-    const bool full = vec16 && a.T % kTermChunk == 0;
+    const bool full = vec16 && a.T % kTermChunk == 0 && !split;
+    if (split) {
+        hipError_t err = launch_zero_dst_rows(a, s);
+        if (err != hipSuccess) return err;
+    }
     if (full && dec)
         hipLaunchKernelGGL((rows_terms_lds_kernel<true, true, true>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (full)

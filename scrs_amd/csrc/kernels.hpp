@@ -36,7 +36,25 @@ struct RowsTermsArgs {
     const uint32_t *term_index = nullptr; // table term of source t (device, [T]), or null: t
     uint64_t L = 0;                   // symbol length in bytes
     uint64_t count = 0;               // codewords
+    // Set by the caller: every destination row address and L are multiples
+    // of 4, so a launch too small to fill the device may split each
+    // codeword's terms into segments whose partial rows are XORed
+    // atomically into destinations cleared first (not with `accumulate`).
+    uint32_t dst_aligned4 = 0;
+    // Set by the launcher.
+    uint32_t segs = 1, seg_terms = 0;
 };
+
+// Term split of a launch too small to fill the device (see RowsTermsArgs):
+// sets segs / seg_terms, true when the launch splits.  Segments are at
+// least kSplitMinTerms terms; the split aims at kSplitFillBlocks workgroups.
+constexpr uint32_t kSplitMinTerms = 8;
+constexpr uint64_t kSplitFillBlocks = 512;
+bool plan_term_split(RowsTermsArgs &a, uint64_t active_blocks);
+// The process-wide switch of the split (scrs_set_term_split), off by default.
+bool term_split_enabled();
+// Clears the destination rows of a split launch (a kernel, no memset).
+hipError_t launch_zero_dst_rows(const RowsTermsArgs &a, hipStream_t s);
 
 // Table path: split-nibble product tables staged in LDS, 16-byte loads.
 // `vec16` requires every row address and L to be multiples of 16.

@@ -19,3 +19,18 @@ def torch_cuda():
     if not torch.cuda.is_available():
         pytest.fail("GPU test collected without a visible GPU")
     return torch
+
+
+@pytest.fixture
+def term_split():
+    """Set the library's term-split switch for one test; restored after."""
+    from scrs_amd.codec import set_term_split
+
+    prev = []
+
+    def set_(enable):
+        prev.append(set_term_split(enable))
+
+    yield set_
+    if prev:
+        set_term_split(prev[0])

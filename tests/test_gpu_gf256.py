@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 import oracle
-from scrs_amd.codec import (BatchDecoder, BatchEncoder, DecodeError, EncodeError, Engine, Profile)
+from scrs_amd.codec import (BatchDecoder, BatchEncoder, DecodeError, EncodeError, Engine, Profile,
+                            fill_random_dev)
 
 pytestmark = pytest.mark.gpu
 
@@ -300,3 +301,31 @@ def test_encoder_shared_across_threads(torch_cuda, engine):
     for t in threads:
         t.join()
     assert errors == []
+
+
+@pytest.mark.parametrize("engine", [Engine.STANDARD_CAUCHY, Engine.GOOD_CAUCHY])
+@pytest.mark.parametrize("k,m,L", [(100, 20, 65536), (40, 7, 4100), (17, 3, 1028), (200, 33, 2048)])
+@pytest.mark.parametrize("split", [False, True])
+def test_lone_codeword_term_split(torch_cuda, engine, k, m, L, split, term_split):
+    """One codeword fills too few workgroups: with the term split enabled its
+    terms run as segments whose partial rows are XORed into cleared
+    destinations; encode and decode against the oracle either way."""
+    torch = torch_cuda
+    term_split(split)
+    if engine == Engine.GOOD_CAUCHY and k + m > 255:
+        k = 255 - m
+    p = Profile.resolve(engine, k, m, L)
+    data = torch.empty(k * L, dtype=torch.uint8, device="cuda")
+    fill_random_dev(data, data.numel(), k * 31 + m)
+    rep = torch.full((m * L,), 0xAB, dtype=torch.uint8, device="cuda")
+    BatchEncoder(p).encode_batch_dev(data, rep, 1)
+    torch.cuda.synchronize()
+    d = data.cpu().numpy()
+    assert np.array_equal(rep.cpu().numpy(), oracle.gf256_encode(oracle.gf256_matrix(int(engine), k, m), k, m, L, d))
+    rng = np.random.default_rng(k + m)
+    erased = set(rng.choice(k, size=min(m, k), replace=False).tolist())
+    idx = np.array([[i for i in range(k + m) if i not in erased][:k]], np.uint16)
+    out = torch.full_like(data, 0xCD)
+    BatchDecoder(p).decode_batch_dev(idx, data, rep, out, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(out, data)

@@ -343,3 +343,28 @@ def test_batch_round_trip_term_chunks(torch_cuda, k, m, L, count):
         d = d_all[c * k * L:(c + 1) * k * L]
         assert np.array_equal(r_all[c * m * L:(c + 1) * m * L], oracle.tower_encode(cmat, k, m, L, d)), c
     assert torch.equal(out, data)
+
+
+@pytest.mark.parametrize("k,m,L", [(150, 30, 8192), (64, 9, 4100), (200, 50, 1024)])
+@pytest.mark.parametrize("split", [False, True])
+def test_lone_codeword_term_split(torch_cuda, k, m, L, split, term_split):
+    """One Tower codeword, with and without the term split of the encode and
+    the finalize (partial rows XORed into cleared destinations)."""
+    torch = torch_cuda
+    term_split(split)
+    from scrs_amd.codec import fill_random_dev
+    p = Profile.resolve(TOWER, k, m, L)
+    data = torch.empty(k * L, dtype=torch.uint8, device="cuda")
+    fill_random_dev(data, data.numel(), k * 7 + m)
+    rep = torch.full((m * L,), 0xAB, dtype=torch.uint8, device="cuda")
+    IncrementalEncoder(p).encode_batch_dev(data, rep, 1)
+    torch.cuda.synchronize()
+    d = data.cpu().numpy()
+    assert np.array_equal(rep.cpu().numpy(), oracle.tower_encode(oracle.tower_cauchy_matrix(k, m), k, m, L, d))
+    rng = np.random.default_rng(k)
+    erased = set(rng.choice(k, size=m, replace=False).tolist())
+    idx = np.array([rng.permutation([i for i in range(k + m) if i not in erased])[:k]], np.uint16)
+    out = torch.full_like(data, 0xCD)
+    BatchDecoder(p).decode_batch_dev(idx, data, rep, out, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(out, data)
