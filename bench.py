@@ -556,8 +556,8 @@ def main() -> None:
         "encode_ms": round(enc_ms, 4),
         "decode_ms": round(dec_ms, 4),
         # The dominant kernel is the decode's multiply-XOR (the longest launch
-        # of the step); its rate is taken over the whole decode phase (solve,
-        # tables and pointer kernels included), timed with events on the
+        # of the step); its rate is taken over the whole decode phase (solve and
+        # pointer kernels included), timed with events on the
         # launch stream.  The encode launch is reported beside it.
         "roofline": {
             "bound": "hbm",

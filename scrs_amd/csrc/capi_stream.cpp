@@ -225,7 +225,9 @@ static scrs_status finalize_dev(scrs_decoder *dec, uint8_t *out_dev, hipStream_t
                                            e.d_rows.as<uint32_t>(), data_base, repair_base, out_dev, s, vec16);
         }
         cache->misses++;
+        dec->eng.write_tables = true;  // the recipe keeps the product tables
         scrs_status r = dec->eng.enqueue(idx.data(), nullptr, data_base, repair_base, out_dev, 1, s, vec16);
+        dec->eng.write_tables = false;
         if (r.code != SCRS_OK) return r;
         cache->lru.emplace_front();
         RecipeEntry &e = cache->lru.front();

@@ -160,10 +160,15 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         pa.groups = groups;
         pa.L = L;
         pa.count = cn;
+        // GF(2^8) table path: the multiply-XOR builds its product tables in
+        // LDS from the fused coefficients, so none are written to HBM.
+        const bool fused_in_rows = !gf16 && !bitmatrix && !write_tables;
         if (bitmatrix) {
             // Bit-matrix operands from the solve; the prepare step then
             // writes only the pointer tables.
             if ((err = launch_bitmatrix_tables(pa, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+            pa.fused = nullptr;
+        } else if (fused_in_rows) {
             pa.fused = nullptr;
         }
         if ((err = launch_decode_prepare(pa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
@@ -179,6 +184,7 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         ra.groups = groups;
         ra.L = L;
         ra.count = cn;
+        if (fused_in_rows) ra.fused = d_fused.as<uint16_t>();
         // Missing rows are out + index * L.
         ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
         err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);

@@ -48,6 +48,9 @@ struct DecodeEngine {
     hipEvent_t idx_copied = nullptr;
     uint64_t cap = 0;     // codewords the workspace holds
     uint64_t chunk = 0;   // codewords per launch (bounded table memory)
+    // GF(2^8) table path: write the product tables to d_tables even though
+    // the multiply-XOR builds its own in LDS (a recipe cache copies them).
+    bool write_tables = false;
 
     DecodeEngine() = default;
     DecodeEngine(const DecodeEngine &) = delete;

@@ -42,6 +42,11 @@ using namespace rowsdev;
 // the decode addressing: per-codeword source / destination pointer tables
 // and verbatim copies of the present data rows; otherwise sources and
 // destinations are strided rows of back-to-back codewords (encode).
+// Four packed GF(2^8) bytes times 2 (x, polynomial 0x11B), byte-wise.
+__device__ __forceinline__ uint32_t xtime4(uint32_t x) {
+    return ((x & 0x7f7f7f7fu) << 1) ^ (((x >> 7) & 0x01010101u) * 0x1bu);
+}
+
 template <bool VEC, bool DEC, bool FULL, int RW>
 __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here,
                                 uint64_t p, bool do_copy, uint32_t tb, uint32_t te) {
@@ -58,12 +63,43 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
     for (uint32_t t0 = tb; t0 < te; t0 += kTermChunk) {
         const uint32_t tc = min(static_cast<uint32_t>(kTermChunk), te - t0);
         __syncthreads();
-        // Stage this chunk's tables: entry n of term t keeps its first RW bytes.
-        for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += kThreads) {
-            const uint32_t t = i / (32 * D), rem = i % (32 * D);
-            const uint32_t n = rem / D, d = rem % D;
-            const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
-            reinterpret_cast<uint32_t *>(lds)[i] = tab_cw[(tt * a.groups + rg) * kGroupDwords + n * 4 + d];
+        if (DEC && a.fused) {
+            // Build this chunk's tables from the fused coefficients instead
+            // of reading them: a thread takes one term, one half (entries n
+            // or n << 4) and one dword (4 rows), forms the products of the 4
+            // packed coefficients with 2^b by byte-wise doubling, and writes
+            // the 16 entries as XORs of those (the layout of the staged
+            // tables below).
+            const uint16_t *fc = a.fused + c * a.rmax * a.T;  // [row][term]
+            const uint32_t rows_c = DEC ? load_uniform(a.rows + c) : a.rmax;
+            for (uint32_t i = threadIdx.x; i < tc * 2 * D; i += kThreads) {
+                const uint32_t t = i / (2 * D), half = (i / D) % 2, d = i % D;
+                uint32_t v = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const uint32_t row = static_cast<uint32_t>(rg) * kRowsPerPass + 4 * d + b;
+                    if (row < rows_c) v |= static_cast<uint32_t>(fc[static_cast<uint64_t>(row) * a.T + t0 + t]) << (8 * b);
+                }
+                if (half) v = xtime4(xtime4(xtime4(xtime4(v))));
+                const uint32_t b[4] = {v, xtime4(v), xtime4(xtime4(v)), xtime4(xtime4(xtime4(v)))};
+                uint32_t *dst = reinterpret_cast<uint32_t *>(lds) + t * 32 * D + half * 16 * D + d;
+                // Gray-code order: each next entry differs by one basis term.
+                uint32_t e = 0;
+                dst[0] = 0;
+#pragma unroll
+                for (int g = 1; g < 16; g++) {
+                    e ^= b[__builtin_ctz(g)];
+                    dst[(g ^ (g >> 1)) * D] = e;
+                }
+            }
+        } else {
+            // Stage this chunk's tables: entry n of term t keeps its first RW bytes.
+            for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += kThreads) {
+                const uint32_t t = i / (32 * D), rem = i % (32 * D);
+                const uint32_t n = rem / D, d = rem % D;
+                const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
+                reinterpret_cast<uint32_t *>(lds)[i] = tab_cw[(tt * a.groups + rg) * kGroupDwords + n * 4 + d];
+            }
         }
         __syncthreads();
         if (!active) continue;
