@@ -78,6 +78,17 @@ uint64_t scrs_device_alloc_count(void);
  * Process-wide; returns the previous setting. */
 int32_t scrs_set_term_split(int32_t enable);
 
+/* Largest number of missing data symbols for which an additive-FFT decode
+ * takes the targeted path: a closed-form solve over the systematic locator
+ * and one multiply-XOR of the k received symbols into the missing ones,
+ * instead of the transforms over the whole domain.  Replaces the constant
+ * TARGETED_MAX_MISSING = 5 of scrs/src/afft/decoder.rs:578 (the same
+ * default here); the decoded bytes are the same on either path, so the
+ * setting only moves the crossover.  Clamped to [0, 64]; 0 sends every
+ * decode through the transforms.  Process-wide; returns the previous
+ * setting. */
+int32_t scrs_afft_set_targeted_max(int32_t max_missing);
+
 /* ---- Profile selection ------------------------------------------------ */
 
 /* Engine::field (codec.rs:42-50).  Returns -1 for an unknown engine. */

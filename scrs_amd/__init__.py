@@ -31,5 +31,6 @@ from .codec import (  # noqa: F401
     now_ns,
     device_alloc_count,
     set_term_split,
+    afft_set_targeted_max,
     recommended_gf16_engine,
 )

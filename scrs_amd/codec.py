@@ -724,6 +724,14 @@ def set_term_split(enable: bool) -> bool:
     return bool(_lib.load().scrs_set_term_split(1 if enable else 0))
 
 
+def afft_set_targeted_max(max_missing: int) -> int:
+    """Largest number of missing data symbols for which an additive-FFT
+    decode takes the targeted path instead of the transforms (process-wide,
+    default 5 as TARGETED_MAX_MISSING in scrs/src/afft/decoder.rs:578,
+    clamped to [0, 64]).  Returns the previous setting."""
+    return int(_lib.load().scrs_afft_set_targeted_max(int(max_missing)))
+
+
 def now_ns() -> int:
     return int(_lib.load().scrs_now_ns())
 

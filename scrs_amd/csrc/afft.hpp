@@ -16,6 +16,7 @@
 
 #include "../../include/scrs.h"
 #include "device.hpp"
+#include "engine.hpp"
 
 namespace scrs {
 
@@ -101,6 +102,16 @@ struct AfftDeviceTables {
 };
 hipError_t afft_device_tables(const AfftDeviceTables **out);
 
+// Decodes with at most this many missing data symbols take the direct path
+// (the reference's targeted path, afft/decoder.rs:360-441): one weighted
+// closed-form solve and one multiply-XOR of the k received rows into the
+// missing ones, instead of the transforms over the whole domain.  The
+// decoded bytes are the same on either path.  kAfftDirectCap bounds the
+// process-wide setting (scrs_afft_set_targeted_max).
+constexpr uint32_t kAfftDirectCap = 64;
+constexpr uint32_t kAfftTargetedDefault = 5;
+uint32_t afft_targeted_max();
+
 // Reed-Solomon codec over GF(2^16) evaluated at the points 0..n-1
 // (points = wire indices, afft/profile.rs:41-46): data symbol i is P(i),
 // repair j is P(k + j), deg P < k.
@@ -118,6 +129,7 @@ struct AfftEngine {
     HostBuf h_bits_pinned;
     hipEvent_t bits_copied = nullptr;
     uint64_t enc_cap = 0;   // codewords the encode workspace holds
+    DecodeEngine direct;    // decode of a few erasures (afft_targeted_max)
     // The workspaces serve one call at a time: calls lock `mu`, and work
     // enqueued on one stream waits for the previous call's work on another.
     std::mutex mu;

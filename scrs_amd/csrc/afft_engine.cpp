@@ -12,6 +12,7 @@
 // the forward transform, and P(e) = (Lambda P)'(e) / Lambda'(e).  The encode
 // of other shapes is the same computation with the data points known and
 // the repair points as targets.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -139,6 +140,21 @@ scrs_status AfftEngine::init(const scrs_profile &p, bool for_decode) {
             (err = work.ensure(N * L)) != hipSuccess || (err = work2.ensure(N * L)) != hipSuccess ||
             (err = hipEventCreateWithFlags(&bits_copied, hipEventDisableTiming)) != hipSuccess)
             return fail(err);
+        // Direct decode weights from the locator of the data points
+        // (afft/decoder.rs:256-267): with the points x >= k known, the
+        // locator constants are P(x) there and 1 / P'(d) at the data
+        // points d < k.
+        const AfftMath &am = afft_math();
+        std::vector<uint8_t> known(N, 0);
+        for (uint64_t x = k; x < N; x++) known[x] = 1;
+        const std::vector<uint16_t> c = afft_locator_scales(known, D);
+        std::vector<uint16_t> wlog(n);
+        for (uint32_t x = 0; x < n; x++) {
+            const uint32_t lg = am.log[c[x]];
+            wlog[x] = static_cast<uint16_t>(x < k ? lg : (kAfftOrder - lg) % kAfftOrder);
+        }
+        scrs_status r = direct.init_afft_direct(p, kAfftDirectCap, wlog);
+        if (r.code != SCRS_OK) return r;
     }
     return ok();
 }
@@ -302,6 +318,11 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
     for (uint32_t t = 0; t < k; t++) h_bits[indices[t] >> 5] |= 1u << (indices[t] & 31u);
     for (uint32_t x = 0; x < k; x++)
         if (!((h_bits[x >> 5] >> (x & 31u)) & 1u)) missing++;
+    if (missing > 0 && missing <= std::min(afft_targeted_max(), direct.rmax)) {
+        auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+        const bool vec16 = !ptrs && L % 16 == 0 && al16(data_base) && al16(repair_base) && al16(out);
+        return direct.enqueue(indices, ptrs, data_base, repair_base, out, 1, s, vec16);
+    }
     const uint64_t words = h_bits.size();
     if ((err = hipEventSynchronize(bits_copied)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     std::memcpy(h_bits_pinned.ptr, h_bits.data(), words * 4);

@@ -1,6 +1,7 @@
 // capi_common.cpp - status names and the additive-FFT hooks.
 #include <atomic>
 
+#include "afft.hpp"
 #include "handles.hpp"
 #include "status.hpp"
 
@@ -9,12 +10,20 @@ extern "C" uint64_t scrs_device_alloc_count(void) { return scrs::alloc_counter()
 namespace scrs {
 namespace {
 std::atomic<int32_t> g_term_split{0};
+std::atomic<int32_t> g_afft_targeted_max{static_cast<int32_t>(kAfftTargetedDefault)};
 }  // namespace
 bool term_split_enabled() { return g_term_split.load(std::memory_order_relaxed) != 0; }
+uint32_t afft_targeted_max() { return static_cast<uint32_t>(g_afft_targeted_max.load(std::memory_order_relaxed)); }
 }  // namespace scrs
 
 extern "C" int32_t scrs_set_term_split(int32_t enable) {
     return scrs::g_term_split.exchange(enable ? 1 : 0, std::memory_order_relaxed);
+}
+
+extern "C" int32_t scrs_afft_set_targeted_max(int32_t max_missing) {
+    const int32_t cap = static_cast<int32_t>(scrs::kAfftDirectCap);
+    const int32_t v = max_missing < 0 ? 0 : (max_missing > cap ? cap : max_missing);
+    return scrs::g_afft_targeted_max.exchange(v, std::memory_order_relaxed);
 }
 
 extern "C" const char *scrs_status_name(int32_t code) {

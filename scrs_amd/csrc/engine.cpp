@@ -1,6 +1,7 @@
 // engine.cpp - device decode engine and coefficient plans.
 #include "engine.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 #include "matrices.hpp"
@@ -74,7 +75,34 @@ scrs_status DecodeEngine::init(const scrs_profile &p) {
     L = p.symbol_len;
     rmax = k < m ? k : m;
     groups = (rmax + 15) / 16;
-    const std::vector<uint16_t> coords = cauchy_coordinates(p);
+    return init_device(cauchy_coordinates(p));
+}
+
+scrs_status DecodeEngine::init_afft_direct(const scrs_profile &p, uint32_t max_missing,
+                                           const std::vector<uint16_t> &wlog) {
+    prof = p;
+    gf16 = true;
+    k = static_cast<uint32_t>(p.k);
+    m = static_cast<uint32_t>(p.m);
+    L = p.symbol_len;
+    rmax = std::min({k, m, max_missing});
+    groups = (rmax + 15) / 16;
+    weighted = true;
+    force_split = true;
+    const uint64_t n = uint64_t{k} + m;
+    if (wlog.size() != n) return st(SCRS_ERR_INVALID_ARGUMENT, 3);
+    std::vector<uint16_t> coords(n);
+    for (uint64_t i = 0; i < n; i++) coords[i] = static_cast<uint16_t>(i);
+    hipError_t err = d_wlog.ensure(n * sizeof(uint16_t));
+    if (err == hipSuccess) err = hipMemcpy(d_wlog.ptr, wlog.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (err != hipSuccess)
+        return err == hipErrorOutOfMemory ? st(SCRS_CONFIG_TOO_MANY_SYMBOLS, engine_capacity(p.engine))
+                                          : st(SCRS_ERR_DEVICE, err);
+    return init_device(coords);
+}
+
+scrs_status DecodeEngine::init_device(const std::vector<uint16_t> &coords) {
+    const scrs_profile &p = prof;
     hipError_t err = d_coords.ensure(coords.size() * sizeof(uint16_t));
     if (err == hipSuccess)
         err = hipMemcpy(d_coords.ptr, coords.data(), coords.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
@@ -106,7 +134,7 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
     hipError_t err;
     if (want > cap) {
         const uint64_t kk = k, rr = rmax ? rmax : 1;
-        const uint64_t work = cauchy_solve_work_elems(k, rmax);
+        const uint64_t work = cauchy_solve_work_elems(k, m, rmax);
         if ((err = d_idx.ensure(want * kk * 2)) != hipSuccess || (err = h_idx.ensure(want * kk * 2)) != hipSuccess ||
             (err = d_fused.ensure(want * rr * kk * 2)) != hipSuccess ||
             (err = d_missing.ensure(want * rr * 2)) != hipSuccess || (err = d_rows.ensure(want * 4)) != hipSuccess ||
@@ -134,12 +162,13 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         sa.missing = d_missing.as<uint16_t>();
         sa.rows = d_rows.as<uint32_t>();
         sa.work = d_work.as<uint16_t>();
-        sa.work_stride = cauchy_solve_work_elems(k, rmax);
+        sa.work_stride = cauchy_solve_work_elems(k, m, rmax);
         sa.error_flag = d_flag.as<uint32_t>();
         sa.k = k;
         sa.m = m;
         sa.rmax = rmax;
         sa.count = cn;
+        if (weighted) sa.point_wlog = d_wlog.as<uint16_t>();
         if ((err = launch_cauchy_solve(sa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
         DecodePrepArgs pa;
         pa.indices = d_idx.as<uint16_t>();
@@ -185,6 +214,7 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         ra.L = L;
         ra.count = cn;
         if (fused_in_rows) ra.fused = d_fused.as<uint16_t>();
+        ra.force_split = force_split;
         // Missing rows are out + index * L.
         ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
         err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);

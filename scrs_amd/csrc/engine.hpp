@@ -51,6 +51,10 @@ struct DecodeEngine {
     // GF(2^8) table path: write the product tables to d_tables even though
     // the multiply-XOR builds its own in LDS (a recipe cache copies them).
     bool write_tables = false;
+    // Weighted solve (CauchySolveArgs::point_wlog) and forced term split:
+    // the additive-FFT code's direct decode of a few erasures.
+    DevBuf d_wlog;
+    bool weighted = false, force_split = false;
 
     DecodeEngine() = default;
     DecodeEngine(const DecodeEngine &) = delete;
@@ -59,6 +63,14 @@ struct DecodeEngine {
 
     // Returns a construction status (TooManySymbols{cap} on device OOM).
     scrs_status init(const scrs_profile &p);
+    // The additive-FFT code's direct decode of at most `max_missing` data
+    // erasures: points are the wire indices, and the generator entry of data
+    // z in repair i is P(i) / ((i + z) P'(z)) for P the product of (x + d)
+    // over the data points d < k; wlog[x] (x < k + m) = log(1 / P'(x)) for
+    // data and log(1 / P(x)) for repairs.
+    scrs_status init_afft_direct(const scrs_profile &p, uint32_t max_missing, const std::vector<uint16_t> &wlog);
+    // Shared tail of the two: coordinates, events, flag, launch chunking.
+    scrs_status init_device(const std::vector<uint16_t> &coords);
     // Enqueue the decode of `count` codewords whose indices (host memory,
     // count x k, received order) are already validated.  Received payloads
     // come from `payload_ptrs` (device array, count x k) or from the

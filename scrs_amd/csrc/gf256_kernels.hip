@@ -16,6 +16,7 @@
 
 #include <algorithm>
 
+#include "gf_device.hpp"
 #include "kernels.hpp"
 #include "rows_device.hpp"
 
@@ -42,10 +43,6 @@ using namespace rowsdev;
 // the decode addressing: per-codeword source / destination pointer tables
 // and verbatim copies of the present data rows; otherwise sources and
 // destinations are strided rows of back-to-back codewords (encode).
-// Four packed GF(2^8) bytes times 2 (x, polynomial 0x11B), byte-wise.
-__device__ __forceinline__ uint32_t xtime4(uint32_t x) {
-    return ((x & 0x7f7f7f7fu) << 1) ^ (((x >> 7) & 0x01010101u) * 0x1bu);
-}
 
 template <bool VEC, bool DEC, bool FULL, int RW>
 __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg, int rows_here,
@@ -80,8 +77,8 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
                     const uint32_t row = static_cast<uint32_t>(rg) * kRowsPerPass + 4 * d + b;
                     if (row < rows_c) v |= static_cast<uint32_t>(fc[static_cast<uint64_t>(row) * a.T + t0 + t]) << (8 * b);
                 }
-                if (half) v = xtime4(xtime4(xtime4(xtime4(v))));
-                const uint32_t b[4] = {v, xtime4(v), xtime4(xtime4(v)), xtime4(xtime4(xtime4(v)))};
+                if (half) v = gf8_xtime4(gf8_xtime4(gf8_xtime4(gf8_xtime4(v))));
+                const uint32_t b[4] = {v, gf8_xtime4(v), gf8_xtime4(gf8_xtime4(v)), gf8_xtime4(gf8_xtime4(gf8_xtime4(v)))};
                 uint32_t *dst = reinterpret_cast<uint32_t *>(lds) + t * 32 * D + half * 16 * D + d;
                 // Gray-code order: each next entry differs by one basis term.
                 uint32_t e = 0;
@@ -290,12 +287,12 @@ __global__ void fill_splitmix_kernel(uint8_t *dst, uint64_t len, uint64_t seed, 
 bool plan_term_split(RowsTermsArgs &a, uint64_t active_blocks) {
     a.segs = 1;
     a.seg_terms = a.T;
-    if (!term_split_enabled() || a.accumulate || !a.dst_aligned4 || a.L % 4 || a.rmax == 0 ||
+    const uint64_t fill = a.force_split ? kForceSplitFillBlocks : kSplitFillBlocks;
+    if (!(term_split_enabled() || a.force_split) || a.accumulate || !a.dst_aligned4 || a.L % 4 || a.rmax == 0 ||
         a.T < 2 * kSplitMinTerms ||
-        active_blocks >= kSplitFillBlocks || a.count > 65535)
+        active_blocks >= fill || a.count > 65535)
         return false;
-    const uint64_t want =
-        std::min<uint64_t>(a.T / kSplitMinTerms, (kSplitFillBlocks + active_blocks - 1) / active_blocks);
+    const uint64_t want = std::min<uint64_t>(a.T / kSplitMinTerms, (fill + active_blocks - 1) / active_blocks);
     if (want < 2) return false;
     a.seg_terms = static_cast<uint32_t>((a.T + want - 1) / want);
     a.segs = (a.T + a.seg_terms - 1) / a.seg_terms;

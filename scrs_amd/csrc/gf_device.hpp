@@ -21,6 +21,11 @@ struct Gf8Tables {
     __device__ __forceinline__ uint32_t inv(uint32_t x) const { return x ? exp[255 - log[x]] : 0u; }
 };
 
+// Four packed GF(2^8) bytes times x (polynomial 0x11B), byte-wise.
+__device__ __forceinline__ uint32_t gf8_xtime4(uint32_t v) {
+    return ((v & 0x7f7f7f7fu) << 1) ^ (((v >> 7) & 0x01010101u) * 0x1bu);
+}
+
 // Field element operations over a uint32_t carrier.
 struct FieldGf256 {
     Gf8Tables t;

@@ -45,6 +45,9 @@ struct RowsTermsArgs {
     // codeword's terms into segments whose partial rows are XORed
     // atomically into destinations cleared first (not with `accumulate`).
     uint32_t dst_aligned4 = 0;
+    // Split the terms even when the process-wide switch is off (a caller
+    // whose launches are always far too small to fill the device).
+    uint32_t force_split = 0;
     // Set by the launcher.
     uint32_t segs = 1, seg_terms = 0;
 };
@@ -54,6 +57,12 @@ struct RowsTermsArgs {
 // least kSplitMinTerms terms; the split aims at kSplitFillBlocks workgroups.
 constexpr uint32_t kSplitMinTerms = 8;
 constexpr uint64_t kSplitFillBlocks = 512;
+// Target of a forced split (force_split): a lone codeword of tens of
+// thousands of terms and a few rows.
+#ifndef SCRS_FORCE_SPLIT_BLOCKS
+#define SCRS_FORCE_SPLIT_BLOCKS 2048
+#endif
+constexpr uint64_t kForceSplitFillBlocks = SCRS_FORCE_SPLIT_BLOCKS;
 bool plan_term_split(RowsTermsArgs &a, uint64_t active_blocks);
 // The process-wide switch of the split (scrs_set_term_split), off by default.
 bool term_split_enabled();
@@ -69,7 +78,14 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &a, bool vec16, hipStream_t
 // missing[c][0..r) ascending, fused[c][j][t] = coefficient of received symbol
 // t for missing row j.  coords[i] is the field coordinate of codeword index
 // i (x for data, y for repairs).  `work` holds cauchy_solve_work_elems(k,
-// rmax) u16 per codeword.
+// m, rmax) u16 per codeword.
+//
+// With point_wlog set (GF(2^16) only), the generator entry of data index z
+// in repair index i is w(z) / (w(i) (coords[i] + coords[z])) for per-index
+// weights w, given as discrete logs point_wlog[i] = log w(i).  The solve is
+// the unweighted one with fused[j][t] scaled by w(indices[t]) / w(missing j):
+// the additive-FFT code's Lagrange generator over the systematic points has
+// this form.
 struct CauchySolveArgs {
     const uint16_t *indices = nullptr;
     const uint16_t *coords = nullptr;
@@ -82,8 +98,9 @@ struct CauchySolveArgs {
     uint32_t k = 0, m = 0, rmax = 0;
     uint64_t count = 0;
     const uint16_t *gf16_log_exp = nullptr;  // filled by the launcher (GF(2^16))
+    const uint16_t *point_wlog = nullptr;    // [k + m] per-index weight logs, or null
 };
-uint64_t cauchy_solve_work_elems(uint32_t k, uint32_t rmax);
+uint64_t cauchy_solve_work_elems(uint32_t k, uint32_t m, uint32_t rmax);
 hipError_t launch_cauchy_solve(const CauchySolveArgs &a, bool gf16, hipStream_t s);
 
 // Product tables and pointer tables from a solve.  GF(2^8) tables use the

@@ -34,3 +34,19 @@ def term_split():
     yield set_
     if prev:
         set_term_split(prev[0])
+
+
+@pytest.fixture
+def afft_targeted():
+    """Set the additive-FFT targeted-path threshold for one test; restored
+    after."""
+    from scrs_amd.codec import afft_set_targeted_max
+
+    prev = []
+
+    def set_(max_missing):
+        prev.append(afft_set_targeted_max(max_missing))
+
+    yield set_
+    if prev:
+        afft_set_targeted_max(prev[0])

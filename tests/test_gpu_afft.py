@@ -265,6 +265,82 @@ def test_full_domain_32768_32768_2048(torch_cuda):
     assert np.array_equal(got, want)
 
 
+def erase_data(rng, k, m, r):
+    """Received indices with exactly r data symbols missing: the first r
+    received repairs replace them (every other repair is absent)."""
+    missing = set(rng.choice(k, size=r, replace=False).tolist())
+    repairs = sorted(rng.choice(m, size=r, replace=False).tolist())
+    idx = [i for i in range(k) if i not in missing] + [k + j for j in repairs]
+    return np.array(rng.permutation(idx), np.uint16)
+
+
+@pytest.mark.parametrize("k,m,L", [(32, 16, 34), (256, 64, 48), (1000, 24, 34), (1500, 548, 66),
+                                   (4096, 4096, 64), (129, 70, 4098)])
+def test_targeted_and_transform_paths_agree(torch_cuda, afft_targeted, k, m, L):
+    """Decodes of r missing data symbols through the targeted path (weighted
+    closed-form solve + multiply-XOR) and through the transforms give the
+    original data, for r up to the targeted path's cap of 64, with the
+    layout and the pointer-array forms (afft/decoder.rs:916-975 covers the
+    two paths at low and high r)."""
+    torch = torch_cuda
+    p = Profile.resolve(AFFT, k, m, L)
+    data = torch.empty(k * L, dtype=torch.uint8, device="cuda")
+    fill_random_dev(data, data.numel(), k * 31 + m)
+    rep = torch.empty(m * L, dtype=torch.uint8, device="cuda")
+    BatchEncoder(p).encode_batch_dev(data, rep, 1)
+    dec = BatchDecoder(p)
+    rng = np.random.default_rng(k + 7 * m)
+    for r in (1, 2, 3, 5, 6, 16, 17, 33, 64):
+        if r > min(k, m):
+            continue
+        idx = erase_data(rng, k, m, r)[None, :]
+        for limit in (0, 64):
+            afft_targeted(limit)
+            out = torch.full_like(data, 0xAB)
+            dec.decode_batch_dev(idx, data, rep, out, 1)
+            torch.cuda.synchronize()
+            assert torch.equal(out, data), (r, limit)
+        ptrs = np.array([data.data_ptr() + int(i) * L if i < k else rep.data_ptr() + (int(i) - k) * L
+                         for i in idx[0]], np.uint64)
+        pt = torch.from_numpy(ptrs.view(np.int64)).cuda()
+        out = torch.full_like(data, 0xCD)
+        dec.decode_batch_ptrs_dev(idx, pt, out, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(out, data), (r, "ptrs")
+
+
+def test_targeted_path_full_domain(torch_cuda, afft_targeted):
+    """k = m = 32768, symbol_len = 2048: 1, 5 and 64 missing data symbols
+    through the targeted path (the term split over 32768 terms), and 5
+    through the transforms, all equal to the original data; the streaming
+    decoder takes the same path."""
+    torch = torch_cuda
+    k, m, L = 32768, 32768, 2048
+    p = Profile.resolve(AFFT, k, m, L)
+    data = torch.empty(k * L, dtype=torch.uint8, device="cuda")
+    fill_random_dev(data, data.numel(), 0x7A6)
+    rep = torch.empty(m * L, dtype=torch.uint8, device="cuda")
+    BatchEncoder(p).encode_batch_dev(data, rep, 1)
+    dec = BatchDecoder(p)
+    rng = np.random.default_rng(0x7A6)
+    afft_targeted(64)
+    for r, limit in ((1, 64), (5, 64), (64, 64), (5, 0)):
+        afft_targeted(limit)
+        idx = erase_data(rng, k, m, r)[None, :]
+        out = torch.full_like(data, 0xAB)
+        dec.decode_batch_dev(idx, data, rep, out, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(out, data), (r, limit)
+
+
+def test_targeted_threshold_setting(torch_cuda, afft_targeted):
+    from scrs_amd.codec import afft_set_targeted_max
+    afft_targeted(5)
+    assert afft_set_targeted_max(1000) == 5
+    assert afft_set_targeted_max(-3) == 64
+    assert afft_set_targeted_max(5) == 0
+
+
 def test_engine_selection_and_errors(torch_cuda):
     with pytest.raises(ConfigError) as e:
         IncrementalEncoder(Profile.resolve(AFFT, 4, 2, 8))
