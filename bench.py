@@ -152,6 +152,9 @@ def gf16_workload(args) -> dict:
     if not torch.equal(out, data):
         raise SystemExit("decode output differs from the original data")
     enc_bytes, dec_bytes = count * (k + m) * sl, count * 2 * k * sl
+    targeted = None
+    if args.workload == "afft":
+        targeted = afft_targeted_line(args, dec, data, rep, out, stream, k, m)
     lds_roof = None
     if args.workload == "tower":
         # The feed reads, per element and term, four nibble-table entries of
@@ -182,8 +185,39 @@ def gf16_workload(args) -> dict:
                      # fraction is of the HBM peak for comparison only.
                      "limiter": "lds"},
         "lds_roofline": lds_roof,
+        "targeted_decode": targeted,
         "cpu_baseline": None,
     }
+
+
+def afft_targeted_line(args, dec, data, rep, out, stream, k, m) -> dict:
+    """AFFT decodes of 5 missing data symbols (the reference's
+    TARGETED_MAX_MISSING) through the targeted path and, for the same
+    pattern, through the transforms; both outputs are checked."""
+    import torch
+
+    from scrs_amd.codec import afft_set_targeted_max
+
+    rng = np.random.default_rng(SEED_ERASE + 5)
+    r = 5
+    missing = set(rng.choice(k, size=r, replace=False).tolist())
+    repairs = sorted(rng.choice(m, size=r, replace=False).tolist())
+    idx = np.array([i for i in range(k) if i not in missing] + [k + j for j in repairs], np.uint16)[None, :]
+    res = {}
+    prev = afft_set_targeted_max(5)
+    try:
+        for name, limit in (("targeted_ms", 5), ("transforms_ms", 0)):
+            afft_set_targeted_max(limit)
+            phase = [lambda: dec.decode_batch_dev(idx, data, rep, out, 1, stream)]
+            for _ in range(args.warmup):
+                phase[0]()
+            _, (ms,) = timed_phases(args.steps, stream, phase)
+            if not torch.equal(out, data):
+                raise SystemExit("targeted-path comparison: decode output differs from the original data")
+            res[name] = round(ms, 4)
+    finally:
+        afft_set_targeted_max(prev)
+    return {"missing": r, **res, "speedup": round(res["transforms_ms"] / res["targeted_ms"], 2)}
 
 
 def serve_workload(args) -> dict:
