@@ -226,9 +226,11 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
         hipError_t err = launch_zero_dst_rows(a, s);
         if (err != hipSuccess) return err;
     }
-    // Term counts that are multiples of 16 or 8 take the unrolled kernels
-    // (whole codewords only: a split launch runs the rolled loop).
-    const int full = (!vec16 || split) ? 0 : (a.T % 16 == 0 ? 16 : (a.T % 8 == 0 ? 8 : 0));
+    // Term counts that are multiples of 16 or 8 take the unrolled kernels,
+    // split launches too when every segment holds whole chunks.
+    const int full = !vec16 ? 0
+                     : (a.T % 16 == 0 && a.seg_terms % 16 == 0 ? 16
+                        : (a.T % 8 == 0 && a.seg_terms % 8 == 0 ? 8 : 0));
     if (full == 16 && dec)
         hipLaunchKernelGGL((rows_terms16_kernel<true, true, 16>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else if (full == 16)

@@ -295,6 +295,10 @@ bool plan_term_split(RowsTermsArgs &a, uint64_t active_blocks) {
     const uint64_t want = std::min<uint64_t>(a.T / kSplitMinTerms, (fill + active_blocks - 1) / active_blocks);
     if (want < 2) return false;
     a.seg_terms = static_cast<uint32_t>((a.T + want - 1) / want);
+    // A forced split keeps whole chunks of 16 (or 8) terms per segment when
+    // the term count allows, so the segments run the unrolled kernels.
+    const uint32_t unit = a.T % 16 == 0 ? 16u : (a.T % 8 == 0 ? 8u : 1u);
+    if (a.force_split) a.seg_terms = (a.seg_terms + unit - 1) / unit * unit;
     a.segs = (a.T + a.seg_terms - 1) / a.seg_terms;
     if (a.segs < 2) {
         a.segs = 1;

@@ -520,7 +520,8 @@ hipError_t launch_decode_prepare(const DecodePrepArgs &a, bool gf16, hipStream_t
     if (a.count == 0) return hipSuccess;
     hipError_t err = ensure_solve_tables();
     if (err != hipSuccess) return err;
-    const uint64_t items = static_cast<uint64_t>(a.k) * a.groups * 16;
+    // Threads per (term, group): 16 rows (GF(2^8)) or 8 row pairs (GF(2^16)).
+    const uint64_t items = static_cast<uint64_t>(a.k) * a.groups * (gf16 ? 8 : 16);
     if (a.count > 65535) return hipErrorInvalidValue;
     // At least 64 workgroups per codeword, more when few codewords would
     // leave the device idle (one codeword of tens of thousands of terms).
