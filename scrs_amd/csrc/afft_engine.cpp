@@ -321,7 +321,10 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
     if (missing > 0 && missing <= std::min(afft_targeted_max(), direct.rmax)) {
         auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
         const bool vec16 = !ptrs && L % 16 == 0 && al16(data_base) && al16(repair_base) && al16(out);
-        return direct.enqueue(indices, ptrs, data_base, repair_base, out, 1, s, vec16);
+        direct.rows_hint = missing;  // the solve's row count is exactly this
+        const scrs_status r = direct.enqueue(indices, ptrs, data_base, repair_base, out, 1, s, vec16);
+        direct.rows_hint = 0;
+        return r;
     }
     const uint64_t words = h_bits.size();
     if ((err = hipEventSynchronize(bits_copied)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);

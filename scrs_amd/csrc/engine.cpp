@@ -215,6 +215,17 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         ra.count = cn;
         if (fused_in_rows) ra.fused = d_fused.as<uint16_t>();
         ra.force_split = force_split;
+        if (force_split && rows_hint && rows_hint <= rmax && k >= 2 * kSplitMinTerms) {
+            // Segments are at most kForceSplitFillBlocks per codeword.
+            const uint64_t need = cn * rows_hint * kForceSplitFillBlocks * L;
+            if (need <= (1ull << 30) && d_partials.ensure(need) == hipSuccess) {
+                ra.partials = d_partials.as<uint8_t>();
+                ra.partial_rows = rows_hint;
+                ra.partials_bytes = need;
+            } else {
+                (void)hipGetLastError();  // a failed allocation only means atomics
+            }
+        }
         // Missing rows are out + index * L.
         ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
         err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);

@@ -55,6 +55,11 @@ struct DecodeEngine {
     // the additive-FFT code's direct decode of a few erasures.
     DevBuf d_wlog;
     bool weighted = false, force_split = false;
+    // Forced splits: the caller's exact row count of the next enqueue (0 =
+    // unknown); with it the split segments store partial rows to d_partials
+    // and a second kernel combines them, instead of atomic XORs.
+    uint32_t rows_hint = 0;
+    DevBuf d_partials;
 
     DecodeEngine() = default;
     DecodeEngine(const DecodeEngine &) = delete;

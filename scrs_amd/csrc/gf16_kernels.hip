@@ -145,6 +145,12 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
 #pragma unroll
         for (int j = 0; j < 4; j++)
             o[j] = ((acc[2 * j][d] >> sh) & 0xffffu) | (((acc[2 * j + 1][d] >> sh) & 0xffffu) << 16);
+        if (a.segs > 1 && a.partials) {  // a term segment: its own partial row
+            if (static_cast<uint32_t>(row) >= a.partial_rows) continue;  // outside the buffer's rows
+            const uint64_t slot = (c * a.partial_rows + row) * a.segs + tb / a.seg_terms;
+            store16<VEC>(a.partials + slot * a.L, p, a.L, o);
+            continue;
+        }
         if (a.segs > 1) {  // a term segment: XOR into the cleared row
             xor16_atomic(dst, p, a.L, o);
             continue;
@@ -169,6 +175,24 @@ __device__ void copy_only16(const RowsTermsArgs &a, uint64_t c, uint64_t p, uint
         load16<VEC>(load_uniform(a.src_ptrs + c * a.T + t), p, a.L, w);
         store16<VEC>(cp, p, a.L, w);
     }
+}
+
+// Second kernel of a split launch with partial rows: dword d of row `row`
+// of codeword blockIdx.z is the XOR of that dword over the segments.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(RowsTermsArgs a) {
+    const uint64_t c = blockIdx.z;
+    const uint32_t row = blockIdx.y;
+    const uint64_t d = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const bool dec = a.dst_ptrs != nullptr;
+    if (d >= a.L / 4 || (dec && row >= load_uniform(a.rows + c))) return;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.partials) +
+                          (c * a.partial_rows + row) * a.segs * (a.L / 4) + d;
+    uint32_t v = 0;
+#pragma unroll 8
+    for (uint32_t sg = 0; sg < a.segs; sg++) v ^= *gbl<uint32_t>(src + static_cast<uint64_t>(sg) * (a.L / 4));
+    uint8_t *dst = dec ? load_uniform(a.dst_ptrs + c * a.rmax + row)
+                       : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(row) * a.L;
+    *gbl_mut<uint32_t>(dst + 4 * d) = v;
 }
 
 template <bool VEC, bool DEC, int FULL>
@@ -222,7 +246,13 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
     if (dec && a.accumulate) return hipErrorInvalidValue;
     const dim3 grid(static_cast<uint32_t>(blocks));
     const uint32_t t32 = static_cast<uint32_t>(tiles), u32 = static_cast<uint32_t>(units);
-    if (split) {
+    // Partial rows when the caller's buffer holds them (rows bounded by
+    // partial_rows, L a multiple of 4 as every split requires).
+    if (!split || a.partial_rows == 0 || a.partial_rows > a.rmax ||
+        a.count * a.partial_rows * a.segs * a.L > a.partials_bytes || (a.L / 4 + 255) / 256 > 0x7fffffffull ||
+        a.count > 65535)
+        a.partials = nullptr;
+    if (split && !a.partials) {
         hipError_t err = launch_zero_dst_rows(a, s);
         if (err != hipSuccess) return err;
     }
@@ -247,6 +277,10 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
         hipLaunchKernelGGL((rows_terms16_kernel<false, true, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else
         hipLaunchKernelGGL((rows_terms16_kernel<false, false, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || !a.partials) return err;
+    const dim3 rgrid(static_cast<uint32_t>((a.L / 4 + 255) / 256), a.partial_rows, static_cast<uint32_t>(a.count));
+    hipLaunchKernelGGL(reduce_partials_kernel, rgrid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -48,6 +48,14 @@ struct RowsTermsArgs {
     // Split the terms even when the process-wide switch is off (a caller
     // whose launches are always far too small to fill the device).
     uint32_t force_split = 0;
+    // GF(2^16) split launches: when set and large enough, every segment
+    // stores its partial rows to partials[c][row][seg][L] (row below
+    // partial_rows, which bounds every codeword's row count) and a second
+    // kernel XORs them into the destinations, instead of atomic XORs into
+    // cleared rows.  partials_bytes is the buffer's size.
+    uint8_t *partials = nullptr;
+    uint32_t partial_rows = 0;
+    uint64_t partials_bytes = 0;
     // Set by the launcher.
     uint32_t segs = 1, seg_terms = 0;
 };
