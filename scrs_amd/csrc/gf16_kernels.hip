@@ -195,6 +195,38 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(RowsTermsArgs a) {
     *gbl_mut<uint32_t>(dst + 4 * d) = v;
 }
 
+// The same for many segments over few dwords (too few threads above to
+// fill the device, each folding thousands of partials in sequence): a
+// workgroup covers 4 dwords with 64 lanes per dword, each lane XORing every
+// 64th segment's partial, and the lanes combine in LDS.
+constexpr uint32_t kReduceDwords = 4, kReduceLanes = 64;
+__global__ __launch_bounds__(256) void reduce_partials_wide_kernel(RowsTermsArgs a) {
+    __shared__ uint32_t part[kReduceLanes][kReduceDwords];
+    const uint64_t c = blockIdx.z;
+    const uint32_t row = blockIdx.y;
+    const uint32_t dw = threadIdx.x % kReduceDwords, lane = threadIdx.x / kReduceDwords;
+    const uint64_t d = static_cast<uint64_t>(blockIdx.x) * kReduceDwords + dw;
+    const uint64_t words = a.L / 4;
+    const bool dec = a.dst_ptrs != nullptr;
+    if (dec && row >= load_uniform(a.rows + c)) return;  // uniform over the workgroup
+    uint32_t v = 0;
+    if (d < words) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(a.partials) + (c * a.partial_rows + row) * a.segs * words + d;
+#pragma unroll 8
+        for (uint32_t sg = lane; sg < a.segs; sg += kReduceLanes) v ^= *gbl<uint32_t>(src + sg * words);
+    }
+    part[lane][dw] = v;
+    __syncthreads();
+    for (uint32_t h = kReduceLanes / 2; h > 0; h >>= 1) {
+        if (lane < h) part[lane][dw] ^= part[lane + h][dw];
+        __syncthreads();
+    }
+    if (lane != 0 || d >= words) return;
+    uint8_t *dst = dec ? load_uniform(a.dst_ptrs + c * a.rmax + row)
+                       : a.dst_base + c * a.dst_cw_stride + static_cast<uint64_t>(row) * a.L;
+    *gbl_mut<uint32_t>(dst + 4 * d) = part[0][dw];
+}
+
 template <bool VEC, bool DEC, int FULL>
 __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a, uint32_t tiles, uint32_t units) {
     __shared__ __align__(16) uint8_t lds[kTermChunk * 64 * 32];
@@ -249,7 +281,7 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
     // Partial rows when the caller's buffer holds them (rows bounded by
     // partial_rows, L a multiple of 4 as every split requires).
     if (!split || a.partial_rows == 0 || a.partial_rows > a.rmax ||
-        a.count * a.partial_rows * a.segs * a.L > a.partials_bytes || (a.L / 4 + 255) / 256 > 0x7fffffffull ||
+        a.count * a.partial_rows * a.segs * a.L > a.partials_bytes || (a.L / 4 + 3) / 4 > 0x7fffffffull ||
         a.count > 65535)
         a.partials = nullptr;
     if (split && !a.partials) {
@@ -279,8 +311,15 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
         hipLaunchKernelGGL((rows_terms16_kernel<false, false, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || !a.partials) return err;
-    const dim3 rgrid(static_cast<uint32_t>((a.L / 4 + 255) / 256), a.partial_rows, static_cast<uint32_t>(a.count));
-    hipLaunchKernelGGL(reduce_partials_kernel, rgrid, dim3(256), 0, s, a);
+    const uint64_t words = a.L / 4;
+    if (a.segs >= 256 && a.count * a.partial_rows * words < 32768) {
+        const dim3 rgrid(static_cast<uint32_t>((words + kReduceDwords - 1) / kReduceDwords), a.partial_rows,
+                         static_cast<uint32_t>(a.count));
+        hipLaunchKernelGGL(reduce_partials_wide_kernel, rgrid, dim3(256), 0, s, a);
+    } else {
+        const dim3 rgrid(static_cast<uint32_t>((words + 255) / 256), a.partial_rows, static_cast<uint32_t>(a.count));
+        hipLaunchKernelGGL(reduce_partials_kernel, rgrid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
