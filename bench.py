@@ -217,7 +217,11 @@ def afft_targeted_line(args, dec, data, rep, out, stream, k, m) -> dict:
             res[name] = round(ms, 4)
     finally:
         afft_set_targeted_max(prev)
-    return {"missing": r, **res, "speedup": round(res["transforms_ms"] / res["targeted_ms"], 2)}
+    # Algorithmic bytes as for every decode: k received symbols read, k
+    # data symbols written.
+    gbs = 2 * k * data.numel() // k / (res["targeted_ms"] / 1e3) / 1e9
+    return {"missing": r, **res, "speedup": round(res["transforms_ms"] / res["targeted_ms"], 2),
+            "targeted_gbs": round(gbs, 2), "targeted_hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def serve_workload(args) -> dict:
