@@ -146,6 +146,17 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
     }
     if ((err = d_tables.ensure(cap * tb)) != hipSuccess)
         return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+    DecodeWorkspace ws;
+    ws.idx = d_idx.as<uint16_t>();
+    ws.fused = d_fused.as<uint16_t>();
+    ws.missing = d_missing.as<uint16_t>();
+    ws.rows = d_rows.as<uint32_t>();
+    ws.work = d_work.as<uint16_t>();
+    ws.src = d_src.as<const uint8_t *>();
+    ws.dst = d_dst.as<uint8_t *>();
+    ws.copy = d_copy.as<uint8_t *>();
+    ws.tables = d_tables.as<uint32_t>();
+    ws.cap = cap;
     for (uint64_t c0 = 0; c0 < count; c0 += cap) {
         const uint64_t cn = count - c0 < cap ? count - c0 : cap;
         // The pinned index buffer is reused: wait for the previous upload.
@@ -155,83 +166,128 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
                 hipSuccess ||
             (err = hipEventRecord(idx_copied, s)) != hipSuccess)
             return st(SCRS_ERR_DEVICE, err);
-        CauchySolveArgs sa;
-        sa.indices = d_idx.as<uint16_t>();
-        sa.coords = d_coords.as<uint16_t>();
-        sa.fused = d_fused.as<uint16_t>();
-        sa.missing = d_missing.as<uint16_t>();
-        sa.rows = d_rows.as<uint32_t>();
-        sa.work = d_work.as<uint16_t>();
-        sa.work_stride = cauchy_solve_work_elems(k, m, rmax);
-        sa.error_flag = d_flag.as<uint32_t>();
-        sa.k = k;
-        sa.m = m;
-        sa.rmax = rmax;
-        sa.count = cn;
-        if (weighted) sa.point_wlog = d_wlog.as<uint16_t>();
-        if ((err = launch_cauchy_solve(sa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
-        DecodePrepArgs pa;
-        pa.indices = d_idx.as<uint16_t>();
-        pa.fused = d_fused.as<uint16_t>();
-        pa.missing = d_missing.as<uint16_t>();
-        pa.rows = d_rows.as<uint32_t>();
-        pa.payload_ptrs = payload_ptrs ? payload_ptrs + c0 * k : nullptr;
-        pa.data_base = data_base ? data_base + c0 * k * L : nullptr;
-        pa.repair_base = repair_base ? repair_base + c0 * m * L : nullptr;
-        pa.out_base = out + c0 * k * L;
-        pa.tables = d_tables.as<uint32_t>();
-        pa.src_ptrs = d_src.as<const uint8_t *>();
-        pa.dst_ptrs = d_dst.as<uint8_t *>();
-        pa.copy_ptrs = d_copy.as<uint8_t *>();
-        pa.k = k;
-        pa.m = m;
-        pa.rmax = rmax;
-        pa.groups = groups;
-        pa.L = L;
-        pa.count = cn;
-        // GF(2^8) table path: the multiply-XOR builds its product tables in
-        // LDS from the fused coefficients, so none are written to HBM.
-        const bool fused_in_rows = !gf16 && !bitmatrix && !write_tables;
-        if (bitmatrix) {
-            // Bit-matrix operands from the solve; the prepare step then
-            // writes only the pointer tables.
-            if ((err = launch_bitmatrix_tables(pa, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
-            pa.fused = nullptr;
-        } else if (fused_in_rows) {
-            pa.fused = nullptr;
-        }
-        if ((err = launch_decode_prepare(pa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
-        RowsTermsArgs ra;
-        ra.src_ptrs = d_src.as<const uint8_t *const>();
-        ra.dst_ptrs = d_dst.as<uint8_t *const>();
-        ra.copy_ptrs = d_copy.as<uint8_t *const>();
-        ra.rows = d_rows.as<uint32_t>();
-        ra.tables = d_tables.as<uint32_t>();
-        ra.table_cw_stride = tb / 4;
-        ra.T = k;
-        ra.rmax = rmax;
-        ra.groups = groups;
-        ra.L = L;
-        ra.count = cn;
-        if (fused_in_rows) ra.fused = d_fused.as<uint16_t>();
-        ra.force_split = force_split;
-        if (force_split && rows_hint && rows_hint <= rmax && k >= 2 * kSplitMinTerms) {
-            // Segments are at most kForceSplitFillBlocks per codeword.
-            const uint64_t need = cn * rows_hint * kForceSplitFillBlocks * L;
-            if (need <= (1ull << 30) && d_partials.ensure(need) == hipSuccess) {
-                ra.partials = d_partials.as<uint8_t>();
-                ra.partial_rows = rows_hint;
-                ra.partials_bytes = need;
-            } else {
-                (void)hipGetLastError();  // a failed allocation only means atomics
-            }
-        }
-        // Missing rows are out + index * L.
-        ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
-        err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);
-        if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        scrs_status r = run_chunk(ws, payload_ptrs ? payload_ptrs + c0 * k : nullptr,
+                                  data_base ? data_base + c0 * k * L : nullptr,
+                                  repair_base ? repair_base + c0 * m * L : nullptr, out + c0 * k * L, cn, s, vec16,
+                                  bitmatrix);
+        if (r.code != SCRS_OK) return r;
     }
     return ok();
+}
+
+static uint64_t round256(uint64_t x) { return (x + 255) & ~uint64_t{255}; }
+
+uint64_t DecodeEngine::workspace_bytes(uint64_t count) const {
+    const uint64_t kk = k, rr = rmax ? rmax : 1;
+    return round256(count * kk * 2) + round256(count * rr * kk * 2) + round256(count * rr * 2) +
+           round256(count * 4) + round256(count * cauchy_solve_work_elems(k, m, rmax) * 2) +
+           round256(count * kk * 8) + round256(count * rr * 8) + round256(count * kk * 8);
+}
+
+DecodeWorkspace DecodeEngine::carve_workspace(uint8_t *base, uint64_t count) const {
+    const uint64_t kk = k, rr = rmax ? rmax : 1;
+    DecodeWorkspace ws;
+    uint8_t *p = base;
+    auto take = [&p](uint64_t bytes) {
+        uint8_t *q = p;
+        p += round256(bytes);
+        return q;
+    };
+    ws.idx = reinterpret_cast<uint16_t *>(take(count * kk * 2));
+    ws.fused = reinterpret_cast<uint16_t *>(take(count * rr * kk * 2));
+    ws.missing = reinterpret_cast<uint16_t *>(take(count * rr * 2));
+    ws.rows = reinterpret_cast<uint32_t *>(take(count * 4));
+    ws.work = reinterpret_cast<uint16_t *>(take(count * cauchy_solve_work_elems(k, m, rmax) * 2));
+    ws.src = reinterpret_cast<const uint8_t **>(take(count * kk * 8));
+    ws.dst = reinterpret_cast<uint8_t **>(take(count * rr * 8));
+    ws.copy = reinterpret_cast<uint8_t **>(take(count * kk * 8));
+    ws.tables = nullptr;
+    ws.cap = count;
+    return ws;
+}
+
+scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *const *payload_ptrs,
+                                    const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out, uint64_t cn,
+                                    hipStream_t s, bool vec16, bool bitmatrix) {
+    const uint64_t tb = cw_table_bytes(bitmatrix);
+    // GF(2^8) table path: the multiply-XOR builds its product tables in LDS
+    // from the fused coefficients, so none are written to HBM.
+    const bool fused_in_rows = !gf16 && !bitmatrix && !write_tables;
+    if (!fused_in_rows && !ws.tables) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    if (cn > ws.cap) return st(SCRS_ERR_CAPACITY, ws.cap);
+    hipError_t err;
+    CauchySolveArgs sa;
+    sa.indices = ws.idx;
+    sa.coords = d_coords.as<uint16_t>();
+    sa.fused = ws.fused;
+    sa.missing = ws.missing;
+    sa.rows = ws.rows;
+    sa.work = ws.work;
+    sa.work_stride = cauchy_solve_work_elems(k, m, rmax);
+    sa.error_flag = d_flag.as<uint32_t>();
+    sa.k = k;
+    sa.m = m;
+    sa.rmax = rmax;
+    sa.count = cn;
+    if (weighted) sa.point_wlog = d_wlog.as<uint16_t>();
+    if ((err = launch_cauchy_solve(sa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    DecodePrepArgs pa;
+    pa.indices = ws.idx;
+    pa.fused = ws.fused;
+    pa.missing = ws.missing;
+    pa.rows = ws.rows;
+    pa.payload_ptrs = payload_ptrs;
+    pa.data_base = data_base;
+    pa.repair_base = repair_base;
+    pa.out_base = out;
+    pa.tables = ws.tables;
+    pa.src_ptrs = ws.src;
+    pa.dst_ptrs = ws.dst;
+    pa.copy_ptrs = ws.copy;
+    pa.k = k;
+    pa.m = m;
+    pa.rmax = rmax;
+    pa.groups = groups;
+    pa.L = L;
+    pa.count = cn;
+    if (bitmatrix) {
+        // Bit-matrix operands from the solve; the prepare step then
+        // writes only the pointer tables.
+        if ((err = launch_bitmatrix_tables(pa, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        pa.fused = nullptr;
+    } else if (fused_in_rows) {
+        pa.fused = nullptr;
+    }
+    if ((err = launch_decode_prepare(pa, gf16, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    RowsTermsArgs ra;
+    ra.src_ptrs = ws.src;
+    ra.dst_ptrs = ws.dst;
+    ra.copy_ptrs = ws.copy;
+    ra.rows = ws.rows;
+    ra.tables = ws.tables;
+    ra.table_cw_stride = tb / 4;
+    ra.T = k;
+    ra.rmax = rmax;
+    ra.groups = groups;
+    ra.L = L;
+    ra.count = cn;
+    if (fused_in_rows) ra.fused = ws.fused;
+    ra.force_split = force_split;
+    // Missing rows are out + index * L.
+    ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
+    if (force_split && gf16 && rows_hint && rows_hint <= rmax && k >= 2 * kSplitMinTerms) {
+        // Sized for the segments this launch actually plans.
+        const uint64_t need = cn * rows_hint * rows_terms16_segments(ra) * L;
+        if (need <= (1ull << 30) && d_partials.ensure(need) == hipSuccess) {
+            ra.partials = d_partials.as<uint8_t>();
+            ra.partial_rows = rows_hint;
+            ra.partials_bytes = need;
+        } else {
+            (void)hipGetLastError();  // a failed allocation only means atomics
+        }
+    }
+    err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);
+    return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }
 
 scrs_status DecodeEngine::enqueue_recipe(const uint16_t *indices, const uint32_t *tables, const uint16_t *missing,

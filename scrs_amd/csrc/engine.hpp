@@ -35,6 +35,21 @@ std::vector<uint32_t> coefficient_tables(const scrs_profile &p, const std::vecto
 // Launch the multiply-XOR of the profile's field.
 hipError_t launch_rows_terms_field(bool gf16, const RowsTermsArgs &a, bool vec16, hipStream_t s);
 
+// Device workspace of one decode launch: received indices, the solve's
+// outputs and work, and the pointer tables, for `cap` codewords.
+struct DecodeWorkspace {
+    uint16_t *idx = nullptr;      // [cap][k]
+    uint16_t *fused = nullptr;    // [cap][rmax][k]
+    uint16_t *missing = nullptr;  // [cap][rmax]
+    uint32_t *rows = nullptr;     // [cap]
+    uint16_t *work = nullptr;     // [cap][cauchy_solve_work_elems]
+    const uint8_t **src = nullptr;  // [cap][k]
+    uint8_t **dst = nullptr;        // [cap][rmax]
+    uint8_t **copy = nullptr;       // [cap][k]
+    uint32_t *tables = nullptr;     // [cap][cw_table_bytes / 4], or null when no kernel reads them
+    uint64_t cap = 0;
+};
+
 // Solve + tables + pointers + multiply-XOR for batches of codewords of one
 // Cauchy geometry (either field).
 struct DecodeEngine {
@@ -85,6 +100,17 @@ struct DecodeEngine {
     scrs_status enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
                         const uint8_t *repair_base, uint8_t *out, uint64_t count, hipStream_t s, bool vec16,
                         bool bitmatrix = false);
+    // The chain of one launch chunk of `cn` codewords whose indices are
+    // already in ws.idx (payload, data, repair and output bases of the
+    // chunk's first codeword).
+    scrs_status run_chunk(const DecodeWorkspace &ws, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
+                          const uint8_t *repair_base, uint8_t *out, uint64_t cn, hipStream_t s, bool vec16,
+                          bool bitmatrix);
+    // Bytes a caller-provided workspace needs for `count` codewords of the
+    // GF(2^8) table kernel (no product tables in HBM): each array starts on
+    // a 256-byte boundary.  carve_workspace lays them out from `base`.
+    uint64_t workspace_bytes(uint64_t count) const;
+    DecodeWorkspace carve_workspace(uint8_t *base, uint64_t count) const;
     // Decode one codeword with a reconstruction recipe computed earlier (its
     // product tables, missing rows and row count on the device): only the
     // pointer tables and the multiply-XOR run.

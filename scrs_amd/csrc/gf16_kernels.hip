@@ -258,6 +258,15 @@ __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a,
 
 }  // namespace
 
+uint32_t rows_terms16_segments(const RowsTermsArgs &args) {
+    RowsTermsArgs a = args;
+    if (a.count == 0 || a.L == 0) return 1;
+    const uint64_t tiles = (a.L + kTile - 1) / kTile;
+    const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
+    plan_term_split(a, a.count * groups * tiles);
+    return a.segs;
+}
+
 hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
     // The 16-byte paths address a row with a 32-bit column offset.
     if (args.L > 0xffffffffull) vec16 = false;

@@ -139,6 +139,9 @@ hipError_t launch_bitmatrix_tables(const DecodePrepArgs &a, hipStream_t s);
 // addressing as RowsTermsArgs; tables in the 2048 B per (term, group) layout
 // (build_nibble_tables16).
 hipError_t launch_rows_terms16(const RowsTermsArgs &a, bool vec16, hipStream_t s);
+// Term segments per codeword that launch_rows_terms16 plans for `a` (1 when
+// it does not split): sizes the partial-row buffer of a split launch.
+uint32_t rows_terms16_segments(const RowsTermsArgs &a);
 
 // Bit-matrix path: the same multiply-XOR as a GF(2) product on the int8
 // matrix cores.  Tables hold, per codeword, row group g and 8-term chunk ch,

@@ -137,7 +137,7 @@ scrs_status scrs_server::plain_one(Request *r, uint64_t *ring_end) {
     while (ring_head + bytes - ring_tail > ring_cap && !inflight.empty()) {
         InFlight &f = inflight.front();
         (void)hipEventSynchronize(f.ev);
-        ring_tail = f.ring_end;
+        if (f.ring_end) ring_tail = f.ring_end;  // encodes hold no ring space
         events.push_back(f.ev);
         finish(f, scrs_now_ns());
         inflight.pop_front();
