@@ -202,6 +202,27 @@ scrs_status scrs_decode_batch_ptrs_dev(scrs_batch_decoder *dec, const uint16_t *
                                        const uint8_t *const *payload_ptrs_dev, uint8_t *out_dev,
                                        uint64_t count, void *stream);
 
+/* Same as scrs_decode_batch_dev, allocating nothing (no reference
+ * counterpart: for callers that cannot allocate during a call, such as code
+ * under HIP graph capture or a tight serving loop).  The launch workspace
+ * (indices, solve outputs, pointer tables) is a fixed per-device array of
+ * scrs_noalloc_scratch_bytes() bytes (32 MiB), held until the process ends;
+ * scrs_decode_batch_dev_noalloc_bytes() gives the bytes a call of `count`
+ * codewords needs.  GF(256) engines only (else UnsupportedMode{engine}), on
+ * the split-nibble table kernel.  Same index checks as scrs_decode_batch_dev;
+ * then count > 65535 or a workspace above the array's size is
+ * SCRS_ERR_CAPACITY{a = limit, b = requested}.  The indices are copied from
+ * host memory before the call returns.  Every call on one device shares
+ * that device's array: calls that overlap on the device (different streams,
+ * or threads) race and must be serialised by the caller. */
+scrs_status scrs_decode_batch_dev_noalloc(scrs_batch_decoder *dec, const uint16_t *indices,
+                                          const uint8_t *data_dev, const uint8_t *repairs_dev, uint8_t *out_dev,
+                                          uint64_t count, void *stream);
+uint64_t scrs_noalloc_scratch_bytes(void);
+/* Workspace bytes of a scrs_decode_batch_dev_noalloc call of `count`
+ * codewords (0 for a null or non-GF(256) decoder). */
+uint64_t scrs_decode_batch_dev_noalloc_bytes(const scrs_batch_decoder *dec, uint64_t count);
+
 /* ---- Decoder: streaming receive (codec.rs:180-201) ---------------------- */
 
 /* decoder(&profile) (selector.rs:117-125).  Device buffers for n received

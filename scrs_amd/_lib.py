@@ -94,6 +94,9 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_decode_into_with": (Status, [H, u64p, C.POINTER(vp), u64p, C.c_uint64, vp, C.c_uint64, H]),
     "scrs_decode_batch_dev": (Status, [H, u16p, vp, vp, vp, C.c_uint64, vp]),
     "scrs_decode_batch_ptrs_dev": (Status, [H, u16p, vp, vp, C.c_uint64, vp]),
+    "scrs_decode_batch_dev_noalloc": (Status, [H, u16p, vp, vp, vp, C.c_uint64, vp]),
+    "scrs_noalloc_scratch_bytes": (C.c_uint64, []),
+    "scrs_decode_batch_dev_noalloc_bytes": (C.c_uint64, [H, C.c_uint64]),
     # streaming decoder
     "scrs_decoder_create": (Status, [C.POINTER(ProfileStruct), HP]),
     "scrs_decoder_destroy": (None, [H]),

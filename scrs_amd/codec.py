@@ -384,6 +384,21 @@ class BatchDecoder(_Coded):
                                                 _dev_ptr(repairs_dev), _dev_ptr(out_dev), count,
                                                 _stream_ptr(stream)))
 
+    def decode_batch_dev_noalloc(self, indices: np.ndarray, data_dev, repairs_dev, out_dev, count: int,
+                                 stream=None) -> None:
+        """decode_batch_dev with its launch workspace in the library's fixed
+        per-device array: the call allocates nothing (GF(256) engines)."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint16)
+        if idx.size != count * self.k:
+            raise ValueError("indices must hold count * k entries")
+        check(_lib.load().scrs_decode_batch_dev_noalloc(self._h, idx.ctypes.data_as(_lib.u16p), _dev_ptr(data_dev),
+                                                        _dev_ptr(repairs_dev), _dev_ptr(out_dev), count,
+                                                        _stream_ptr(stream)))
+
+    def noalloc_bytes(self, count: int) -> int:
+        """Workspace bytes of a decode_batch_dev_noalloc call of `count` codewords."""
+        return int(_lib.load().scrs_decode_batch_dev_noalloc_bytes(self._h, count))
+
     def decode_batch_ptrs_dev(self, indices: np.ndarray, payload_ptrs_dev, out_dev, count: int,
                               stream=None) -> None:
         idx = np.ascontiguousarray(indices, dtype=np.uint16)
@@ -710,6 +725,11 @@ class Server(_Handle):
             c = out[i]
             res[i] = (c.status.code, c.t_submit_ns, c.t_start_ns, c.t_done_ns)
         return res
+
+
+def noalloc_scratch_bytes() -> int:
+    """Size of the per-device workspace array of the no-allocation decode."""
+    return int(_lib.load().scrs_noalloc_scratch_bytes())
 
 
 def device_alloc_count() -> int:
