@@ -227,6 +227,192 @@ def afft_targeted_line(args, dec, data, rep, out, stream, k, m) -> dict:
             "targeted_gbs": round(gbs, 2), "targeted_hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+SEED_SERVE = 0x5C25_0005
+SERVE_LENS = (512, 1500, 4096, 65536)
+
+
+def serving_trace(n_req: int, seed: int = SEED_SERVE) -> list[tuple]:
+    """The mixed serving stream, from one seed: per request
+    (is_encode, engine, k, m, symbol_len, received indices or None).
+    70% encode / 30% decode, engine uniform over {Standard Cauchy, Good
+    Cauchy, Tower}, k ~ U[4,200], m ~ U[1,32], symbol_len from
+    {512, 1500, 4096, 65536} (all even, as GF(2^16) needs), and a decode
+    erases r ~ U[1,m] of its k+m indices and receives k survivors in a
+    random order."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n_req):
+        enc = bool(rng.random() < 0.7)
+        eng = int(rng.integers(0, 3))
+        k, m = int(rng.integers(4, 201)), int(rng.integers(1, 33))
+        sl = int(rng.choice(SERVE_LENS))
+        idx = None
+        if not enc:
+            r = int(rng.integers(1, m + 1))
+            keep = np.ones(k + m, bool)
+            keep[rng.choice(k + m, size=r, replace=False)] = False
+            idx = rng.permutation(np.flatnonzero(keep))[:k].astype(np.uint16)
+        specs.append((enc, eng, k, m, sl, idx))
+    return specs
+
+
+def _align256(x: int) -> int:
+    return (x + 255) // 256 * 256
+
+
+class ServingSetup:
+    """Device buffers and jobs for a serving trace.
+
+    Encode jobs read k*L seeded bytes at a seeded offset of a shared pool.
+    Every decode job reads a real codeword: its data symbols sit in
+    ``ddata`` and its repairs in ``drep``, encoded beforehand.  Each job
+    writes its own 256-byte-aligned output region of one of ``sets`` output
+    sets; decode outputs are laid out exactly like the codewords' data in
+    ``ddata`` (zero padding in both), so one comparison checks every decode.
+    """
+
+    POOL = 1 << 30
+
+    def __init__(self, specs, sets: int = 2, seed: int = SEED_DATA):
+        import torch
+
+        from scrs_amd import Engine, Profile
+
+        self.specs = specs
+        engines = [Engine.STANDARD_CAUCHY, Engine.GOOD_CAUCHY, Engine.TOWER]
+        rng = np.random.default_rng(seed ^ 0x0FF5E7)
+        self.profiles, self.src_off, self.out_off, self.rep_off = [], [], [], []
+        eo = do = ro = 0
+        self.enc_bytes = self.dec_bytes = 0
+        for enc, eng, k, m, sl, idx in specs:
+            self.profiles.append(Profile.resolve(engines[eng], k, m, sl))
+            if enc:
+                self.src_off.append(int(rng.integers(0, (self.POOL - k * sl) // 256)) * 256)
+                self.out_off.append(eo)
+                self.rep_off.append(None)
+                eo += _align256(m * sl)
+                self.enc_bytes += (k + m) * sl
+            else:
+                self.src_off.append(None)
+                self.out_off.append(do)
+                self.rep_off.append(ro)
+                do += _align256(k * sl)
+                ro += _align256(m * sl)
+                self.dec_bytes += 2 * k * sl
+        self.algorithmic_bytes = self.enc_bytes + self.dec_bytes
+        dev = "cuda"
+        self.pool = torch.empty(self.POOL, dtype=torch.uint8, device=dev)
+        self.ddata = torch.zeros(max(do, 256), dtype=torch.uint8, device=dev)
+        self.drep = torch.zeros(max(ro, 256), dtype=torch.uint8, device=dev)
+        self.out_e = [torch.zeros(max(eo, 256), dtype=torch.uint8, device=dev) for _ in range(sets)]
+        self.out_d = [torch.zeros(max(do, 256), dtype=torch.uint8, device=dev) for _ in range(sets)]
+        from scrs_amd import fill_random_dev
+
+        fill_random_dev(self.pool, self.POOL, seed)
+        # Seeded bytes in each decode codeword's data rows, zeros between.
+        for i, (enc, _e, k, _m, sl, _idx) in enumerate(specs):
+            if not enc:
+                o = self.out_off[i]
+                fill_random_dev(self.ddata[o:], k * sl, seed + 1 + i)
+        torch.cuda.synchronize()
+
+    def _encode_job(self, i, out_set):
+        from scrs_amd import Job
+
+        return Job.encode(self.profiles[i], self.pool.data_ptr() + self.src_off[i],
+                          self.out_e[out_set].data_ptr() + self.out_off[i])
+
+    def encode_codewords(self) -> None:
+        """Repairs of every decode job's codeword, by one batch-engine
+        submission (checked against the oracle by sample_encodes)."""
+        import torch
+
+        from scrs_amd import BatchEngine, Job
+
+        jobs = []
+        for i, (enc, _e, k, _m, sl, _idx) in enumerate(self.specs):
+            if not enc:
+                jobs.append(Job.encode(self.profiles[i], self.ddata.data_ptr() + self.out_off[i],
+                                       self.drep.data_ptr() + self.rep_off[i]))
+        if jobs:
+            eng = BatchEngine(tile_bytes=4096, cw_per_unit=1, use_graph=False)
+            eng.submit(jobs)
+            torch.cuda.synchronize()
+            eng.close()
+
+    def jobs(self, out_set: int) -> list:
+        from scrs_amd import Job
+
+        out = []
+        for i, (enc, _e, k, _m, sl, idx) in enumerate(self.specs):
+            if enc:
+                out.append(self._encode_job(i, out_set))
+                continue
+            base_d = self.ddata.data_ptr() + self.out_off[i]
+            base_r = self.drep.data_ptr() + self.rep_off[i]
+            ptrs = [base_d + int(x) * sl if x < k else base_r + (int(x) - k) * sl for x in idx]
+            out.append(Job.decode(self.profiles[i], idx, ptrs, self.out_d[out_set].data_ptr() + self.out_off[i]))
+        return out
+
+    def decodes_ok(self, out_set: int) -> bool:
+        """Every decode output equals its codeword's data (one comparison)."""
+        import torch
+
+        return bool(torch.equal(self.out_d[out_set], self.ddata))
+
+    def sets_equal(self, a: int, b: int) -> bool:
+        import torch
+
+        return bool(torch.equal(self.out_e[a], self.out_e[b]) and torch.equal(self.out_d[a], self.out_d[b]))
+
+    def clear(self, out_set: int) -> None:
+        self.out_e[out_set].zero_()
+        self.out_d[out_set].zero_()
+
+    def sample_encodes(self, out_set: int, per_engine: int, seed: int = 0x5A3) -> int:
+        """Compare `per_engine` seeded encode jobs of each engine, and as
+        many pre-encoded decode codewords, with the oracle (GF(256) Cauchy
+        product or GF(2^16) Tower product over the same input bytes).
+        Raises SystemExit on a mismatch; returns the jobs checked."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        import oracle
+
+        rng = np.random.default_rng(seed)
+        picks = []
+        for eng in range(3):
+            enc_i = [i for i, s in enumerate(self.specs) if s[0] and s[1] == eng]
+            dec_i = [i for i, s in enumerate(self.specs) if not s[0] and s[1] == eng]
+            for group in (enc_i, dec_i):
+                if group:
+                    n = min(per_engine if group is enc_i else max(1, per_engine // 4), len(group))
+                    picks += [int(x) for x in rng.choice(group, size=n, replace=False)]
+        work = []
+        for i in picks:
+            enc, eng, k, m, sl, _idx = self.specs[i]
+            if enc:
+                src = self.pool[self.src_off[i]:self.src_off[i] + k * sl]
+                rep = self.out_e[out_set][self.out_off[i]:self.out_off[i] + m * sl]
+            else:
+                src = self.ddata[self.out_off[i]:self.out_off[i] + k * sl]
+                rep = self.drep[self.rep_off[i]:self.rep_off[i] + m * sl]
+            work.append((i, eng, k, m, sl, src.cpu().numpy(), rep.cpu().numpy()))
+
+        def check(w):
+            i, eng, k, m, sl, src, rep = w
+            if eng == 2:
+                want = oracle.tower_encode(oracle.tower_cauchy_matrix(k, m), k, m, sl, src)
+            else:
+                want = oracle.gf256_encode(oracle.gf256_matrix(eng, k, m), k, m, sl, src)
+            return i if not np.array_equal(want, rep) else None
+
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            bad = [i for i in ex.map(check, work) if i is not None]
+        if bad:
+            raise SystemExit(f"serving outputs differ from the oracle for requests {bad[:8]}")
+        return len(work)
+
+
 def serve_workload(args) -> dict:
     """Mixed serving stream: 20000 requests, 70% encode / 30% decode, engine
     uniform over {Standard Cauchy, Good Cauchy, Tower}, k ~ U[4,200],
