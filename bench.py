@@ -48,44 +48,75 @@ def erasure_indices(count: int, k: int, m: int, max_r: int, seed: int) -> np.nda
     return idx
 
 
+def cpu_threads() -> tuple[int, dict]:
+    """Threads for the CPU baselines: the CPU share this process may use
+    (its affinity set, capped by OMP_NUM_THREADS where the host sets one, as
+    a GPU box does per GPU), and a description of the host."""
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(share, int(omp)) if omp.isdigit() and int(omp) > 0 else share
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, {"cpu_model": model, "host_cpus": os.cpu_count(), "affinity_cpus": share,
+                     "omp_num_threads": omp or None}
+
+
+def _timed_reps(fn, target_s: float, min_reps: int = 5) -> tuple[float, float, int]:
+    """Run fn() once to warm up, then at least min_reps times and until
+    target_s has passed; returns (min, median, reps) seconds per call."""
+    fn()
+    times = []
+    t_end = time.perf_counter() + target_s
+    while len(times) < min_reps or time.perf_counter() < t_end:
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return min(times), float(np.median(times)), len(times)
+
+
 def cpu_baseline(idx: np.ndarray, rep_gpu_sample: np.ndarray, target_s: float = 10.0) -> dict:
     """The oracle's register-blocked GFNI port, timed on this host on a
     bounded sample of the same workload; its outputs are checked against the
     GPU's for the sampled codewords."""
     import oracle
 
-    sample = 16
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, host = cpu_threads()
+    sample = max(16, threads)
     coeffs = oracle.good_cauchy_matrix(K, M)
     data = oracle.splitmix_bytes_np(SEED_DATA, 0, sample * K * L)
-    t0 = time.perf_counter()
     rep = oracle.gf256_encode_many(coeffs, K, M, L, data, sample, threads)
     out = oracle.gf256_decode_many(coeffs, K, M, L, idx[:sample], data, rep, sample, threads)
-    one = time.perf_counter() - t0
-    if not np.array_equal(rep, rep_gpu_sample):
+    if not np.array_equal(rep[:rep_gpu_sample.size], rep_gpu_sample):
         raise SystemExit("CPU and GPU repairs differ on the sampled codewords")
     if not np.array_equal(out, data):
         raise SystemExit("CPU decode did not round-trip")
-    reps = max(1, int(target_s / max(one, 1e-3)))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        rep = oracle.gf256_encode_many(coeffs, K, M, L, data, sample, threads)
-        oracle.gf256_decode_many(coeffs, K, M, L, idx[:sample], data, rep, sample, threads)
-    dt = time.perf_counter() - t0
-    nbytes = reps * sample * ((K + M) * L + 2 * K * L)
+
+    def step():
+        r = oracle.gf256_encode_many(coeffs, K, M, L, data, sample, threads)
+        oracle.gf256_decode_many(coeffs, K, M, L, idx[:sample], data, r, sample, threads)
+
+    t_min, t_med, reps = _timed_reps(step, target_s)
+    nbytes = sample * ((K + M) * L + 2 * K * L)
     # The reference is single-threaded: one core on a smaller sample too.
-    t0 = time.perf_counter()
-    one_rep = oracle.gf256_encode_many(coeffs, K, M, L, data[: 2 * K * L], 2, 1)
-    oracle.gf256_decode_many(coeffs, K, M, L, idx[:2], data[: 2 * K * L], one_rep, 2, 1)
-    dt1 = time.perf_counter() - t0
+    t1, _, _ = _timed_reps(lambda: (oracle.gf256_decode_many(
+        coeffs, K, M, L, idx[:2], data[: 2 * K * L],
+        oracle.gf256_encode_many(coeffs, K, M, L, data[: 2 * K * L], 2, 1), 2, 1)), 1.0, 3)
     plan = {2: "GFNI+AVX2", 1: "AVX2 nibble", 0: "scalar"}[oracle.cpu_plan()]
     return {
-        "value": round(nbytes / dt / 1e9, 3),
+        "value": round(nbytes / t_min / 1e9, 3),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample} codewords x {reps} reps of the same encode+decode ({plan}, one codeword per thread)",
-        "single_core_value": round(2 * ((K + M) * L + 2 * K * L) / dt1 / 1e9, 3),
+        "sample": f"{sample} codewords of the workload, encode + decode, one codeword per thread ({plan}); "
+                  f"best of {reps} runs (median {round(nbytes / t_med / 1e9, 3)} GB/s)",
+        "single_core_value": round(2 * ((K + M) * L + 2 * K * L) / t1 / 1e9, 3),
+        **host,
     }
 
 
@@ -108,7 +139,71 @@ def timed_phases(steps, stream, phases):
     return wall, per
 
 
-def gf16_workload(args) -> dict:
+def tower_cpu_baseline(k, m, sl, idx, data_gpu, rep_gpu, out_gpu, target_s: float = 10.0) -> dict:
+    """The reference's Tower CPU path, ported (oracle/scrs_cpu16.c): the
+    GFNI byte-swap rows kernel for the incremental feed and the
+    rational-Lagrange finalize, one codeword per thread, timed on a bounded
+    sample of the workload's codewords; the sample's repairs and decoded data
+    are checked against the GPU's."""
+    import oracle
+
+    threads, host = cpu_threads()
+    sample = max(8, threads)
+    data = data_gpu[: sample * k * sl].cpu().numpy()
+    coeffs = oracle.tower_cauchy_matrix(k, m)
+    rep = oracle.tower_encode_port(coeffs, k, m, sl, data, count=sample, threads=threads)
+    if not np.array_equal(rep, rep_gpu[: sample * m * sl].cpu().numpy()):
+        raise SystemExit("CPU and GPU Tower repairs differ on the sampled codewords")
+    out = oracle.tower_finalize_port(k, m, sl, idx[:sample], data, rep, count=sample, threads=threads)
+    if out is None or not np.array_equal(out, out_gpu[: sample * k * sl].cpu().numpy()):
+        raise SystemExit("CPU and GPU Tower finalize outputs differ on the sampled codewords")
+
+    def step():
+        r = oracle.tower_encode_port(coeffs, k, m, sl, data, count=sample, threads=threads)
+        oracle.tower_finalize_port(k, m, sl, idx[:sample], data, r, count=sample, threads=threads)
+
+    t_min, t_med, reps = _timed_reps(step, target_s)
+    nbytes = sample * ((k + m) * sl + 2 * k * sl)
+    plan = "GFNI+AVX2" if oracle.cpu16_plan() == 2 else "scalar"
+    return {"value": round(nbytes / t_min / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} of the workload's codewords, incremental feed + finalize, one codeword per "
+                      f"thread ({plan}); best of {reps} runs (median {round(nbytes / t_med / 1e9, 3)} GB/s)",
+            **host}
+
+
+def afft_cpu_baseline(k, m, sl, idx, data_gpu, rep_gpu, target_s: float = 10.0) -> dict:
+    """The reference's additive-FFT CPU path, ported (oracle/scrs_cpu16.c):
+    the strip-blocked fused encode (strips shared out over the threads) and
+    the complete erasure-locator decode (column slices over the threads), on
+    the whole workload; outputs are checked against the GPU's."""
+    import oracle
+
+    threads, host = cpu_threads()
+    data = data_gpu.cpu().numpy()
+    rep_g = rep_gpu.cpu().numpy()
+    rep, width = oracle.afft_encode_port(k, m, sl, data, threads=threads)
+    if not np.array_equal(rep, rep_g):
+        raise SystemExit("CPU and GPU additive-FFT repairs differ")
+    sym = [(i, data[i * sl:(i + 1) * sl] if i < k else rep_g[(i - k) * sl:(i - k + 1) * sl])
+           for i in (int(x) for x in idx[0])]
+    out = oracle.afft_decode_port(k, m, sl, sym, threads=threads)
+    if not np.array_equal(out, data):
+        raise SystemExit("CPU additive-FFT decode did not reconstruct the data")
+    e_min, e_med, e_reps = _timed_reps(lambda: oracle.afft_encode_port(k, m, sl, data, threads=threads, out=rep),
+                                       target_s / 2)
+    d_min, d_med, d_reps = _timed_reps(lambda: oracle.afft_decode_port(k, m, sl, sym, threads=threads),
+                                       target_s / 2)
+    nbytes = (k + m) * sl + 2 * k * sl
+    plan = "GFNI+AVX2" if oracle.cpu16_plan() == 2 else "scalar"
+    return {"value": round(nbytes / (e_min + d_min) / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"the whole workload: fused strip-blocked encode ({width}-byte strips, strips over the "
+                      f"threads) + complete decode (column slices over the threads), {plan}; best of "
+                      f"{e_reps}/{d_reps} runs",
+            "encode_ms": round(e_min * 1e3, 2), "decode_ms": round(d_min * 1e3, 2),
+            "median_encode_ms": round(e_med * 1e3, 2), "median_decode_ms": round(d_med * 1e3, 2), **host}
+
+
+def gf16_workload(args, which: str | None = None) -> dict:
     """GF(2^16) lines: Tower k=200 m=50 symbol_len=8192 x 256 codewords
     (incremental feed of all k, then finalize with exactly 50 data symbols
     erased), or AFFT k=m=32768 symbol_len=2048 (one 128 MiB codeword, decode
@@ -117,9 +212,10 @@ def gf16_workload(args) -> dict:
 
     from scrs_amd import Engine, Profile, batch_decoder, batch_encoder, fill_random_dev, incremental_encoder
 
+    which = which or args.workload
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
-    if args.workload == "tower":
+    if which == "tower":
         k, m, sl, count = 200, 50, 8192, 256
         prof = Profile.resolve(Engine.TOWER, k, m, sl)
         enc = incremental_encoder(prof)
@@ -153,10 +249,10 @@ def gf16_workload(args) -> dict:
         raise SystemExit("decode output differs from the original data")
     enc_bytes, dec_bytes = count * (k + m) * sl, count * 2 * k * sl
     targeted = None
-    if args.workload == "afft":
+    if which == "afft":
         targeted = afft_targeted_line(args, dec, data, rep, out, stream, k, m)
     lds_roof = None
-    if args.workload == "tower":
+    if which == "tower":
         # The feed reads, per element and term, four nibble-table entries of
         # each 16-row group: 2 bytes per row of the group, 8 bytes for groups
         # of at most 4 rows and 16 for at most 8 (the staged entry sizes).
@@ -167,6 +263,11 @@ def gf16_workload(args) -> dict:
         lds_roof = {"kernel": "rows_terms16_kernel, feed", "achieved": round(lds_tbs, 2),
                     "peak": LDS_PEAK_TBS, "unit": "TB/s", "frac": round(lds_tbs / LDS_PEAK_TBS, 4),
                     "lds_bytes_per_launch": lds_bytes}
+    cpu = None
+    if not args.no_cpu:
+        torch.cuda.synchronize()
+        cpu = (tower_cpu_baseline(k, m, sl, idx, data, rep, out) if which == "tower"
+               else afft_cpu_baseline(k, m, sl, idx, data, rep))
     value = (enc_bytes + dec_bytes) / (wall / args.steps) / 1e9
     enc_gbs = enc_bytes / (enc_ms / 1e3) / 1e9
     dec_gbs = dec_bytes / (dec_ms / 1e3) / 1e9
@@ -186,7 +287,7 @@ def gf16_workload(args) -> dict:
                      "limiter": "lds"},
         "lds_roofline": lds_roof,
         "targeted_decode": targeted,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -413,76 +514,96 @@ class ServingSetup:
         return len(work)
 
 
-def serve_workload(args) -> dict:
+def serve_cpu_baseline(setup: "ServingSetup", target_s: float = 10.0, sample: int = 512) -> dict:
+    """The reference's CPU paths on a seeded sample of the trace's requests
+    (GF(256): the GFNI register-blocked multiply-XOR encode and recipe
+    decode; Tower: the GFNI byte-swap feed and rational-Lagrange finalize),
+    one request per thread; every sampled output is checked against the
+    GPU's batched output."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+
+    threads, host = cpu_threads()
+    rng = np.random.default_rng(0xC9B)
+    picks = sorted(int(i) for i in rng.choice(len(setup.specs), size=min(sample, len(setup.specs)), replace=False))
+    work, nbytes = [], 0
+    for i in picks:
+        enc, eng, k, m, sl, idx = setup.specs[i]
+        if enc:
+            src = setup.pool[setup.src_off[i]:setup.src_off[i] + k * sl].cpu().numpy()
+            want = setup.out_e[0][setup.out_off[i]:setup.out_off[i] + m * sl].cpu().numpy()
+            work.append((True, eng, k, m, sl, None, src, None, want))
+            nbytes += (k + m) * sl
+        else:
+            d = setup.ddata[setup.out_off[i]:setup.out_off[i] + k * sl].cpu().numpy()
+            r = setup.drep[setup.rep_off[i]:setup.rep_off[i] + m * sl].cpu().numpy()
+            want = setup.out_d[0][setup.out_off[i]:setup.out_off[i] + k * sl].cpu().numpy()
+            work.append((False, eng, k, m, sl, np.ascontiguousarray(idx, np.uint16)[None, :], d, r, want))
+            nbytes += 2 * k * sl
+    mats = {}
+
+    def matrix(eng, k, m):
+        key = (eng, k, m)
+        if key not in mats:
+            mats[key] = (oracle.tower_cauchy_matrix(k, m) if eng == 2 else oracle.gf256_matrix(eng, k, m))
+        return mats[key]
+
+    for w in work:
+        matrix(w[1], w[2], w[3])
+
+    def one(w):
+        enc, eng, k, m, sl, idx, a, b, _want = w
+        c = mats[(eng, k, m)]
+        if eng == 2:
+            return (oracle.tower_encode_port(c, k, m, sl, a) if enc
+                    else oracle.tower_finalize_port(k, m, sl, idx, a, b))
+        return (oracle.gf256_encode_many(c, k, m, sl, a, 1, 1) if enc
+                else oracle.gf256_decode_many(c, k, m, sl, idx, a, b, 1, 1))
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        got = list(ex.map(one, work))
+        bad = [picks[i] for i, (g, w) in enumerate(zip(got, work)) if g is None or not np.array_equal(g, w[8])]
+        if bad:
+            raise SystemExit(f"CPU port and GPU serving outputs differ for requests {bad[:8]}")
+        t_min, t_med, reps = _timed_reps(lambda: list(ex.map(one, work)), target_s)
+    return {"value": round(nbytes / t_min / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{len(work)} seeded requests of the trace, one request per thread (GF(256) GFNI "
+                      f"multiply-XOR / recipe decode, Tower GFNI feed / rational-Lagrange finalize); "
+                      f"best of {reps} passes (median {round(nbytes / t_med / 1e9, 3)} GB/s)",
+            "req_per_s": round(len(work) / t_min, 1), **host}
+
+
+def serve_workload(args, bounded: bool = False) -> dict:
     """Mixed serving stream: 20000 requests, 70% encode / 30% decode, engine
     uniform over {Standard Cauchy, Good Cauchy, Tower}, k ~ U[4,200],
     m ~ U[1,32], symbol_len from {512, 1500, 4096, 65536}, erasures ~ U[1,m],
     one seed.  The same trace runs through the serving loop in plain mode
     (one codec call per request) and in batched mode (continuous batches
-    through the fused engine, hipGraph replay, autotuned tiles)."""
+    through the fused engine, hipGraph replay, autotuned tiles).  Every
+    decode output is compared with its codeword's data, every batched output
+    with the plain one, and a seeded sample of encodes with the oracle.
+    `bounded` trims the latency sweep (the sub-object of the default line)."""
     import torch
 
-    from scrs_amd import Engine, Job, Profile, Server, fill_random_dev, set_term_split
+    from scrs_amd import Server, set_term_split
 
     torch.cuda.set_device(0)
     n_req = args.requests
-    rng = np.random.default_rng(0x5C25_0005)
-    engines = [Engine.STANDARD_CAUCHY, Engine.GOOD_CAUCHY, Engine.TOWER]
-    pool_bytes = 8 << 30
-    pool = torch.empty(pool_bytes, dtype=torch.uint8, device="cuda")
-    fill_random_dev(pool, pool_bytes, SEED_DATA)
-    specs, out_bytes, total_alg = [], 0, 0
-    for _ in range(n_req):
-        enc = rng.random() < 0.7
-        eng = engines[int(rng.integers(0, 3))]
-        k, m = int(rng.integers(4, 201)), int(rng.integers(1, 33))
-        sl = int(rng.choice([512, 1500, 4096, 65536]))
-        prof = Profile.resolve(eng, k, m, sl)
-        off = int(rng.integers(0, (pool_bytes - k * sl) // 256)) * 256
-        if enc:
-            specs.append((True, prof, off, None, None, out_bytes))
-            out_bytes += (m * sl + 255) // 256 * 256
-            total_alg += (k + m) * sl
-        else:
-            roff = int(rng.integers(0, (pool_bytes - m * sl) // 256)) * 256
-            r = int(rng.integers(1, m + 1))
-            erased = set(rng.choice(k + m, size=r, replace=False).tolist())
-            surv = [x for x in range(k + m) if x not in erased]
-            idx = [int(x) for x in rng.permutation(surv)[:k]]
-            specs.append((False, prof, off, roff, idx, out_bytes))
-            out_bytes += (k * sl + 255) // 256 * 256
-            total_alg += 2 * k * sl
-    outs = torch.empty(out_bytes, dtype=torch.uint8, device="cuda")
-    base, obase = pool.data_ptr(), outs.data_ptr()
-    jobs = []
-    for enc, prof, off, roff, idx, o in specs:
-        sl, k = prof.symbol_len, prof.k
-        if enc:
-            jobs.append(Job.encode(prof, base + off, obase + o))
-        else:
-            ptrs = [base + off + x * sl if x < k else base + roff + (x - k) * sl for x in idx]
-            jobs.append(Job.decode(prof, idx, ptrs, obase + o))
-    sample = list(range(0, n_req, max(1, n_req // 64)))
+    specs = serving_trace(n_req)
+    setup = ServingSetup(specs, sets=2)
+    setup.encode_codewords()
+    total_alg = setup.algorithmic_bytes
+    prepared = [Server.prepare(setup.jobs(0)), Server.prepare(setup.jobs(1))]
 
-    def snapshot():
-        torch.cuda.synchronize()
-        res = []
-        for i in sample:
-            enc, prof, _, _, _, o = specs[i]
-            nb = (prof.m if enc else prof.k) * prof.symbol_len
-            res.append(outs[o:o + nb].cpu())
-        return res
-
-    prepared = Server.prepare(jobs)
-
-    def run(srv, batched: bool, rate: float | None):
+    def run(srv, out_set: int, rate: float | None, label: str):
         if rate is None:
             arr = np.zeros(n_req, np.uint64)
         else:
             gaps = np.random.default_rng(0xA55).exponential(1e9 / rate, size=n_req)
             arr = np.cumsum(gaps).astype(np.uint64)
         before = srv.stats()
-        res = srv.replay(prepared, arr)
+        res = srv.replay(prepared[out_set], arr)
         after = srv.stats()
         if (res["code"] != 0).any():
             bad = np.flatnonzero(res["code"] != 0)
@@ -497,85 +618,97 @@ def serve_workload(args) -> dict:
         out["mean_batch_jobs"] = round((after["jobs"] - before["jobs"]) / nb, 1)
         out["host_us_per_batch"] = round((after["host_ns"] - before["host_ns"]) / nb / 1e3, 1)
         out["gpu_us_per_batch"] = round((after["gpu_ns"] - before["gpu_ns"]) / nb / 1e3, 1)
-        print(f"serve {'batched' if batched else 'plain'} rate={rate}: {out}", file=sys.stderr, flush=True)
+        print(f"serve {label} rate={rate}: {out}", file=sys.stderr, flush=True)
         return out
+
+    def verify(out_set: int, what: str):
+        torch.cuda.synchronize()
+        if not setup.decodes_ok(out_set):
+            raise SystemExit(f"{what}: a decode did not reconstruct its codeword's data")
 
     # Each server first replays the whole trace once untimed: the plain path's
     # codecs and the batched path's resident tables, recipes and tuned
     # configurations are then warm, as in a long-running server.
     plain = Server(batched=False)
-    run(plain, False, None)
-    plain_sat = run(plain, False, None)
-    ref = snapshot()
-    outs.fill_(0)
+    run(plain, 1, None, "plain warm-up")
+    verify(1, "plain")
+    checked = setup.sample_encodes(1, per_engine=24 if bounded else 70)
+    setup.clear(1)
+    plain_sat = run(plain, 1, None, "plain")
+    verify(1, "plain")
     batched = Server(batched=True, max_batch_jobs=args.max_batch, tile_bytes=0, cw_per_unit=0, use_graph=True,
                      cache_bytes=16 << 30)
-    run(batched, True, None)
-    batched_sat = run(batched, True, None)
-    got = snapshot()
-    if any(not torch.equal(a, b) for a, b in zip(ref, got)):
+    run(batched, 0, None, "batched warm-up")
+    setup.clear(0)
+    batched_sat = run(batched, 0, None, "batched")
+    verify(0, "batched")
+    if not setup.sets_equal(0, 1):
         raise SystemExit("batched and per-request outputs differ")
-    # Equal offered load at half the plain path's capacity, then the batched
-    # path at the highest load whose p99 stays within the plain path's p99.
-    load = 0.5 * plain_sat["req_per_s"]
-    plain_half = run(plain, False, load)
-    batched_half = run(batched, True, load)
-    best = None
-    for mult in (2, 4, 6, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96):
-        rate = load * mult
-        if rate > 1.2 * batched_sat["req_per_s"]:
-            break
-        rr = run(batched, True, rate)
-        if rr["p99_ms"] > plain_half["p99_ms"]:
-            break
-        best = dict(rr, offered_req_per_s=round(rate, 1))
-    ratio = round(best["req_per_s"] / plain_half["req_per_s"], 2) if best else None
+    cpu = None if args.no_cpu else serve_cpu_baseline(setup)
+
+    def sweep(plain_srv, plain_capacity, mults, label):
+        """Plain at half its capacity; then the batched path at the highest
+        load whose p99 stays within the plain path's p99 there."""
+        load = 0.5 * plain_capacity
+        p_half = run(plain_srv, 1, load, f"{label} half load")
+        best = None
+        for mult in mults:
+            rate = load * mult
+            if rate > 1.2 * batched_sat["req_per_s"]:
+                break
+            rr = run(batched, 0, rate, "batched")
+            if rr["p99_ms"] > p_half["p99_ms"]:
+                break
+            best = dict(rr, offered_req_per_s=round(rate, 1))
+        ratio = round(best["req_per_s"] / p_half["req_per_s"], 2) if best else None
+        return p_half, best, ratio
+
+    mults = (2, 4, 8, 16, 32, 64) if bounded else (2, 4, 6, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96)
+    plain_half, best, ratio = sweep(plain, plain_sat["req_per_s"], mults, "plain")
+    batched_half = run(batched, 0, 0.5 * plain_sat["req_per_s"], "batched at plain half load")
     # The plain path again with the codec's term split enabled, so that a
     # lone codeword's terms spread over the device; the same comparison
     # against it is reported beside the main one.
     prev_split = set_term_split(True)
-    plain_ts = Server(batched=False)
-    run(plain_ts, False, None)
-    plain_ts_sat = run(plain_ts, False, None)
-    got_ts = snapshot()
-    if any(not torch.equal(a, b) for a, b in zip(ref, got_ts)):
-        raise SystemExit("term-split and per-request outputs differ")
-    load_ts = 0.5 * plain_ts_sat["req_per_s"]
-    plain_ts_half = run(plain_ts, False, load_ts)
-    best_ts = None
-    for mult in (1, 1.5, 2, 3, 4, 6, 8, 12, 16, 24, 32):
-        rate = load_ts * mult
-        if rate > 1.2 * batched_sat["req_per_s"]:
-            break
-        rr = run(batched, True, rate)
-        if rr["p99_ms"] > plain_ts_half["p99_ms"]:
-            break
-        best_ts = dict(rr, offered_req_per_s=round(rate, 1))
-    set_term_split(prev_split)
-    plain_ts.close()
+    try:
+        plain_ts = Server(batched=False)
+        run(plain_ts, 1, None, "plain term-split warm-up")
+        setup.clear(1)
+        plain_ts_sat = run(plain_ts, 1, None, "plain term-split")
+        verify(1, "plain term-split")
+        if not setup.sets_equal(0, 1):
+            raise SystemExit("term-split and batched outputs differ")
+        ts_mults = (1, 2, 4, 8, 16) if bounded else (1, 1.5, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+        plain_ts_half, best_ts, ratio_ts = sweep(plain_ts, plain_ts_sat["req_per_s"], ts_mults, "plain term-split")
+        plain_ts.close()
+    finally:
+        set_term_split(prev_split)
     plain.close()
     batched.close()
-    ratio_ts = round(best_ts["req_per_s"] / plain_ts_half["req_per_s"], 2) if best_ts else None
     value = batched_sat["gbs"]
     return {
         "metric": "encode/decode GB/s (bit-exact), % HBM roofline; batched vs plain GPU; p99 latency",
-        "value": value, "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "value": value, "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": round(total_alg / (value * 1e9) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8/u16",
-        "data": "synthetic (seeded splitmix64 bytes in HBM; seeded request trace)",
+        "data": "synthetic (seeded splitmix64 bytes in HBM; seeded request trace; decodes read real codewords)",
         "config": {"workload": "serving stream: %d requests, Poisson arrivals, 70%% encode / 30%% decode, "
                                "engine uniform {Standard, Good, Tower}, k~U[4,200], m~U[1,32], "
                                "symbol_len in {512,1500,4096,65536}, erasures~U[1,m]" % n_req,
-                   "requests": n_req, "max_batch_jobs": args.max_batch},
+                   "requests": n_req, "max_batch_jobs": args.max_batch,
+                   "algorithmic_bytes": total_alg},
+        "checks": {"decodes_vs_data": n_req - sum(s[0] for s in specs), "batched_vs_plain": n_req,
+                   "encodes_vs_oracle": checked},
         "plain_saturated": plain_sat, "batched_saturated": batched_sat,
-        "equal_load": {"offered_req_per_s": round(load, 1), "plain": plain_half, "batched": batched_half},
+        "equal_load": {"offered_req_per_s": round(0.5 * plain_sat["req_per_s"], 1), "plain": plain_half,
+                       "batched": batched_half},
         "batched_at_plain_p99": best, "throughput_ratio_at_equal_p99": ratio,
         "plain_term_split": {"saturated": plain_ts_sat, "equal_load": plain_ts_half,
                              "batched_at_its_p99": best_ts, "throughput_ratio_at_equal_p99": ratio_ts},
         "roofline": {"bound": "hbm", "kernel": "batched serving (whole stream)", "achieved": value,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(value / HBM_PEAK_GBS, 4),
                      "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -634,6 +767,9 @@ def main() -> None:
                          "serve: the mixed request stream, plain vs batched")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests in the trace")
     ap.add_argument("--max-batch", type=int, default=1024, help="serve: jobs per batch at most")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="default workload: print only the GF(256) line, without the Tower, additive-FFT "
+                         "and serving sub-objects")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="before the timed steps, run a plain 1 GiB device copy (torch copy_) so a "
                          "counter profile can scale FETCH_SIZE / WRITE_SIZE by a kernel of known traffic")
@@ -648,7 +784,24 @@ def main() -> None:
     if args.workload != "gf256":
         print(json.dumps(gf16_workload(args)))
         return
+    line = gf256_line(args)
+    if not args.headline_only:
+        # The other configurations, each with its own roofline and CPU
+        # baseline, as sub-objects of the one line.
+        import torch
 
+        for name in ("tower", "afft"):
+            torch.cuda.empty_cache()
+            print(f"bench: {name} workload", file=sys.stderr, flush=True)
+            line[name] = gf16_workload(args, name)
+        torch.cuda.empty_cache()
+        print("bench: serve workload", file=sys.stderr, flush=True)
+        line["serve"] = serve_workload(args, bounded=True)
+    print(json.dumps(line))
+
+
+def gf256_line(args) -> dict:
+    """The headline line: GF(256) Good Cauchy k=32 m=16 L=65536 x 4096."""
     import torch
 
     from scrs_amd import Engine, Profile, batch_decoder, batch_encoder, fill_random_dev
@@ -813,7 +966,8 @@ def main() -> None:
         "bitmatrix_kernel": bitmatrix,
         "host_slice": host,
     }
-    print(json.dumps(line))
+    del data, rep, out
+    return line
 
 
 if __name__ == "__main__":

@@ -81,6 +81,20 @@ def lib() -> C.CDLL:
         L.or_rs16_interp.restype = C.c_int
         L.or_rs16_interp.argtypes = [C.c_size_t, C.c_size_t, _szp, C.POINTER(C.c_void_p), C.c_size_t, _szp, _u8p]
         L.or_afft_encode.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _u8p, _u8p]
+        L.or_cpu16_plan.restype = C.c_int
+        L.or_tower_encode_port.argtypes = [_u16p, C.c_size_t, C.c_size_t, C.c_size_t, _u8p, _u8p]
+        L.or_tower_finalize_port.restype = C.c_int
+        L.or_tower_finalize_port.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _szp, C.POINTER(C.c_void_p), _u8p]
+        L.or_tower_encode_port_many.argtypes = [_u16p, C.c_size_t, C.c_size_t, C.c_size_t, _u8p, _u8p,
+                                                C.c_size_t, C.c_int]
+        L.or_tower_finalize_port_many.restype = C.c_int
+        L.or_tower_finalize_port_many.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _u16p, _u8p, _u8p, _u8p,
+                                                  C.c_size_t, C.c_int]
+        L.or_afft_encode_port.restype = C.c_int
+        L.or_afft_encode_port.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _u8p, _u8p, C.c_int, C.c_size_t]
+        L.or_afft_decode_port.restype = C.c_int
+        L.or_afft_decode_port.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _szp, C.POINTER(C.c_void_p), _u8p,
+                                          C.c_int]
         _lib = L
     return _lib
 
@@ -321,3 +335,63 @@ def splitmix_bytes_np(seed: int, byte_offset: int, length: int) -> np.ndarray:
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         z = z ^ (z >> np.uint64(31))
     return z.astype("<u8").view(np.uint8)[:length].copy()
+
+
+# ---------------------------------------------------------------------------
+# CPU ports of the reference's GF(2^16) paths (the CPU baseline)
+# ---------------------------------------------------------------------------
+
+
+def cpu16_plan() -> int:
+    """2 when the AVX2+GFNI kernels run, 0 for the scalar fallback."""
+    return lib().or_cpu16_plan()
+
+
+def tower_encode_port(coeffs: np.ndarray, k: int, m: int, L: int, data: np.ndarray, count: int = 1,
+                      threads: int = 1) -> np.ndarray:
+    """Tower incremental encode of `count` codewords (the reference's GFNI
+    byte-swap rows kernel), one codeword per thread at a time."""
+    c = np.ascontiguousarray(coeffs, dtype=np.uint16)
+    d = np.ascontiguousarray(data, dtype=np.uint8)
+    out = np.empty(count * m * L, np.uint8)
+    lib().or_tower_encode_port_many(_p16(c), k, m, L, _p8(d), _p8(out), count, threads)
+    return out
+
+
+def tower_finalize_port(k: int, m: int, L: int, idx: np.ndarray, data: np.ndarray, repairs: np.ndarray,
+                        count: int = 1, threads: int = 1) -> np.ndarray | None:
+    """Tower finalize of `count` codewords from idx (count x k received
+    indices; data symbols from `data`, repairs from `repairs`): the
+    rational-Lagrange recipe and output-major apply.  None on failure."""
+    ix = np.ascontiguousarray(idx, dtype=np.uint16)
+    d = np.ascontiguousarray(data, dtype=np.uint8)
+    r = np.ascontiguousarray(repairs, dtype=np.uint8)
+    out = np.empty(count * k * L, np.uint8)
+    ok = lib().or_tower_finalize_port_many(k, m, L, _p16(ix), _p8(d), _p8(r), _p8(out), count, threads)
+    return out if ok else None
+
+
+def afft_encode_port(k: int, m: int, L: int, data: np.ndarray, threads: int = 1, max_strips: int = 0,
+                     out: np.ndarray | None = None) -> tuple[np.ndarray, int]:
+    """Strip-blocked fused additive-FFT encode (k a power of two, m <= k).
+    Returns (repairs, strip width); with max_strips only the first strips'
+    columns are computed."""
+    d = np.ascontiguousarray(data, dtype=np.uint8)
+    if out is None:
+        out = np.zeros(m * L, np.uint8)
+    w = lib().or_afft_encode_port(k, m, L, _p8(d), _p8(out), threads, max_strips)
+    if w < 0:
+        raise ValueError("geometry is off the fused encode path")
+    return out, w
+
+
+def afft_decode_port(k: int, m: int, L: int, symbols, threads: int = 1) -> np.ndarray:
+    """Complete erasure-locator additive-FFT decode of k (index, payload)
+    symbols, column slices shared out over `threads`."""
+    idx, ptrs, lens, keep = _sym_arrays(symbols)
+    if len(symbols) != k or any(n != L for n in lens[:len(symbols)]) or any(i >= k + m for i in idx[:k]):
+        raise ValueError("need k symbols of symbol_len bytes with indices below k + m")
+    out = np.empty(k * L, np.uint8)
+    if lib().or_afft_decode_port(k, m, L, idx, ptrs, _p8(out), threads) != 0:
+        raise ValueError("bad geometry")
+    return out

@@ -120,6 +120,21 @@ uint64_t or_splitmix64(uint64_t *state);
  * with the word counter starting at `seed` + word index (counter mode). */
 void or_fill_bytes(uint64_t seed, uint64_t byte_offset, uint8_t *out, size_t len);
 
+/* ---- CPU ports of the reference's GF(2^16) paths (scrs_cpu16.c) ---- */
+int or_cpu16_plan(void); /* 2: AVX2+GFNI, 0: scalar */
+void or_tower_encode_port(const uint16_t *coeffs, size_t k, size_t m, size_t L, const uint8_t *data,
+                          uint8_t *repairs);
+int or_tower_finalize_port(size_t k, size_t m, size_t L, const size_t *idx, const uint8_t *const *payloads,
+                           uint8_t *out);
+void or_tower_encode_port_many(const uint16_t *coeffs, size_t k, size_t m, size_t L, const uint8_t *data,
+                               uint8_t *repairs, size_t count, int threads);
+int or_tower_finalize_port_many(size_t k, size_t m, size_t L, const uint16_t *idx, const uint8_t *data,
+                                const uint8_t *repairs, uint8_t *out, size_t count, int threads);
+int or_afft_encode_port(size_t k, size_t m, size_t L, const uint8_t *data, uint8_t *repairs, int threads,
+                        size_t max_strips);
+int or_afft_decode_port(size_t k, size_t m, size_t L, const size_t *idx, const uint8_t *const *payloads,
+                        uint8_t *out, int threads);
+
 #ifdef __cplusplus
 }
 #endif
