@@ -67,6 +67,14 @@ const char *scrs_status_name(int32_t code);
  * unchanged. */
 uint64_t scrs_device_alloc_count(void);
 
+/* Heap allocations (operator new, of any size) made by the library's own
+ * code since it was loaded (no reference counterpart).  The host side of the
+ * same steady state: the reference's counting global allocator
+ * (tests/zero_alloc.rs:13-46) sees no allocation in the second block, and
+ * neither does this count.  Allocations inside the HIP runtime are not the
+ * library's and are not counted. */
+uint64_t scrs_host_alloc_count(void);
+
 /* Term split of small codec launches (no reference counterpart).  When
  * enabled, a codec launch too small to fill the device (a lone codeword,
  * say) splits each codeword's terms across workgroups that XOR their

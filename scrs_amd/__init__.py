@@ -30,6 +30,7 @@ from .codec import (  # noqa: F401
     fill_random_dev,
     now_ns,
     device_alloc_count,
+    host_alloc_count,
     set_term_split,
     afft_set_targeted_max,
     recommended_gf16_engine,

@@ -67,6 +67,7 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_abi_version": (C.c_uint32, []),
     "scrs_status_name": (C.c_char_p, [C.c_int32]),
     "scrs_device_alloc_count": (C.c_uint64, []),
+    "scrs_host_alloc_count": (C.c_uint64, []),
     "scrs_set_term_split": (C.c_int32, [C.c_int32]),
     "scrs_afft_set_targeted_max": (C.c_int32, [C.c_int32]),
     "scrs_engine_field": (C.c_int32, [C.c_int32]),

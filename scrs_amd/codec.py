@@ -737,6 +737,11 @@ def device_alloc_count() -> int:
     return int(_lib.load().scrs_device_alloc_count())
 
 
+def host_alloc_count() -> int:
+    """Heap allocations (operator new) the library's own code has made so far."""
+    return int(_lib.load().scrs_host_alloc_count())
+
+
 def set_term_split(enable: bool) -> bool:
     """Let codec launches too small to fill the device split each codeword's
     terms across workgroups (process-wide, off by default).  Returns the

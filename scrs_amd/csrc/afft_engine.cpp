@@ -32,14 +32,23 @@ uint32_t log2_ceil(uint64_t x) {
     return d;
 }
 
+// Consecutive bit ranges (first bit, count) of a transform's passes, held
+// inline: a codec call plans its passes without touching the heap.
+struct BitParts {
+    std::pair<uint32_t, uint32_t> p[kAfftMaxLog];
+    uint32_t n = 0;
+    uint32_t size() const { return n; }
+    const std::pair<uint32_t, uint32_t> &operator[](uint32_t i) const { return p[i]; }
+};
+
 // Split bits [0, D) into ceil(D / cap) nearly equal consecutive parts.
-std::vector<std::pair<uint32_t, uint32_t>> plan_bits(uint32_t D, uint32_t cap) {
-    std::vector<std::pair<uint32_t, uint32_t>> parts;
+BitParts plan_bits(uint32_t D, uint32_t cap) {
+    BitParts parts;
     const uint32_t s = (D + cap - 1) / cap;
     uint32_t lo = 0;
-    for (uint32_t i = 0; i < s; i++) {
+    for (uint32_t i = 0; i < s && i < kAfftMaxLog; i++) {
         const uint32_t g = D / s + (i < D % s ? 1 : 0);
-        parts.emplace_back(lo, g);
+        parts.p[parts.n++] = {lo, g};
         lo += g;
     }
     return parts;

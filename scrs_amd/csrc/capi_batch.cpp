@@ -379,8 +379,10 @@ static scrs_status decode_host(scrs_batch_decoder *dec, const uint64_t *indices,
         return st(SCRS_DECODE_WRONG_COUNT, k, count);
     }
     if (!indices || !payloads || !payload_lens) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
-    std::vector<uint64_t> seen((n + 63) / 64, 0);
-    std::vector<uint16_t> idx16(k);
+    std::vector<uint64_t> &seen = sc.seen;
+    std::vector<uint16_t> &idx16 = sc.idx16;
+    seen.assign((n + 63) / 64, 0);
+    idx16.resize(k);
     for (uint64_t t = 0; t < count; t++) {
         const uint64_t i = indices[t];
         if (i >= n) return st(SCRS_DECODE_INDEX_OUT_OF_RANGE, i, n);
@@ -414,7 +416,8 @@ static scrs_status decode_host(scrs_batch_decoder *dec, const uint64_t *indices,
         return st(SCRS_ERR_DEVICE, err);
     hipStream_t s = dec->stream;
     for (uint64_t t = 0; t < k; t++) std::memcpy(sc.h_payload.as<uint8_t>() + t * L, payloads[t], L);
-    std::vector<const uint8_t *> ptrs(k);
+    std::vector<const uint8_t *> &ptrs = sc.ptrs;
+    ptrs.resize(k);
     for (uint64_t t = 0; t < k; t++) ptrs[t] = sc.d_payload.as<uint8_t>() + t * L;
     if ((err = hipMemcpyAsync(sc.d_payload.ptr, sc.h_payload.ptr, k * L, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (err = hipMemcpyAsync(sc.d_ptrs.ptr, ptrs.data(), k * sizeof(void *), hipMemcpyHostToDevice, s)) !=
@@ -461,9 +464,8 @@ static scrs_status decode_batch(scrs_batch_decoder *dec, const uint16_t *indices
     if (count == 0) return ok();
     if (!indices) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     if (!out_dev) return st(SCRS_ERR_INVALID_ARGUMENT, 5);
-    std::vector<uint64_t> seen;
     for (uint64_t c = 0; c < count; c++) {
-        scrs_status s = check_indices_u16(indices + c * k, k, n, seen);
+        scrs_status s = check_indices_u16(indices + c * k, k, n, dec->seen);
         if (s.code != SCRS_OK) return s;
     }
     if (dec->prof.engine == SCRS_ENGINE_AFFT) {

@@ -41,7 +41,10 @@ extern "C" scrs_status scrs_decoder_create(const scrs_profile *profile, scrs_dec
     (void)hipGetDevice(&dec->device);
     const uint64_t n = p.k + p.m, L = p.symbol_len;
     dec->bits.assign((n + 63) / 64, 0);
+    dec->idx.reserve(p.k);
+    dec->key.reserve(dec->bits.size() + 3);
     hipError_t err = hipStreamCreateWithFlags(&dec->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&dec->order_ev, hipEventDisableTiming);
     if (err == hipSuccess) err = dec->d_slots.ensure(n * L);
     if (err == hipSuccess) err = dec->h_slots.ensure(n * L);
     if (err != hipSuccess) {
@@ -67,6 +70,7 @@ extern "C" void scrs_decoder_destroy(scrs_decoder *dec) {
         (void)hipStreamSynchronize(dec->stream);
         (void)hipStreamDestroy(dec->stream);
     }
+    if (dec->order_ev) (void)hipEventDestroy(dec->order_ev);
     if (dec->afft) scrs_afft_engine_destroy(dec->afft);
     delete dec;
 }
@@ -133,13 +137,8 @@ extern "C" scrs_status scrs_decoder_push_dev(scrs_decoder *dec, uint64_t index, 
     hipStream_t cs = static_cast<hipStream_t>(stream);
     hipError_t err = hipMemcpyAsync(dec->d_slots.as<uint8_t>() + index * L, payload_dev, L, hipMemcpyDeviceToDevice, cs);
     // Finalize orders itself after the decoder's stream: chain the copy there.
-    hipEvent_t ev;
-    if (err == hipSuccess) err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (err == hipSuccess) {
-        err = hipEventRecord(ev, cs);
-        if (err == hipSuccess) err = hipStreamWaitEvent(dec->stream, ev, 0);
-        (void)hipEventDestroy(ev);
-    }
+    if (err == hipSuccess) err = hipEventRecord(dec->order_ev, cs);
+    if (err == hipSuccess) err = hipStreamWaitEvent(dec->stream, dec->order_ev, 0);
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }
 
@@ -192,27 +191,24 @@ extern "C" scrs_status scrs_decoder_scratch_stats(const scrs_decoder_scratch *sc
 // stores the new recipe (decoder/streaming.rs:134-161, decoder/cache.rs).
 static scrs_status finalize_dev(scrs_decoder *dec, uint8_t *out_dev, hipStream_t s, scrs_decoder_scratch *cache) {
     const uint64_t k = dec->prof.k, n = k + dec->prof.m, L = dec->prof.symbol_len;
-    std::vector<uint16_t> idx;
-    idx.reserve(k);
+    std::vector<uint16_t> &idx = dec->idx;  // capacity k from creation
+    idx.clear();
     for (uint64_t i = 0; i < n && idx.size() < k; i++)
         if (bit(dec->bits, i)) idx.push_back(static_cast<uint16_t>(i));
     // Slots are laid out as one codeword: data rows then repair rows.
     const uint8_t *data_base = dec->d_slots.as<uint8_t>();
     const uint8_t *repair_base = data_base + k * L;
     // Host pushes were queued on the decoder's stream: order them first.
-    hipEvent_t ev;
-    hipError_t err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
-    err = hipEventRecord(ev, dec->stream);
-    if (err == hipSuccess) err = hipStreamWaitEvent(s, ev, 0);
-    (void)hipEventDestroy(ev);
+    hipError_t err = hipEventRecord(dec->order_ev, dec->stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s, dec->order_ev, 0);
     if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     if (dec->afft) return scrs_afft_decode(dec->afft, idx.data(), nullptr, data_base, repair_base, out_dev, 1, s);
     const bool vec16 = L % 16 == 0 && (reinterpret_cast<uintptr_t>(out_dev) & 15u) == 0;
     const bool cacheable = cache && engine_is_gf256(dec->prof.engine);
     if (cacheable) {
         // Key: engine, k, m and the receipt pattern (decoder/recipe.rs:9-23).
-        std::vector<uint64_t> key = dec->bits;
+        std::vector<uint64_t> &key = dec->key;  // capacity reserved at creation
+        key.assign(dec->bits.begin(), dec->bits.end());
         key.push_back(static_cast<uint64_t>(dec->prof.engine));
         key.push_back(dec->prof.k);
         key.push_back(dec->prof.m);
@@ -359,6 +355,7 @@ extern "C" void scrs_incr_encoder_destroy(scrs_incr_encoder *enc) {
         (void)hipStreamSynchronize(enc->stream);
         (void)hipStreamDestroy(enc->stream);
     }
+    if (enc->order_ev) (void)hipEventDestroy(enc->order_ev);
     delete enc;
 }
 
@@ -382,8 +379,10 @@ static scrs_status ensure_feed_state(scrs_incr_encoder *enc) {
     if (err == hipSuccess) err = enc->h_pending.ensure(kPending * L);
     if (err == hipSuccess) err = enc->d_ptab.ensure(kPending * 4);
     if (err == hipSuccess) err = enc->h_repairs.ensure(m * L);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&enc->order_ev, hipEventDisableTiming);
     if (err == hipSuccess) err = hipStreamSynchronize(enc->stream);
     if (err != hipSuccess) return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+    enc->pending.reserve(kPending);
     enc->feed_ready = true;
     return ok();
 }
@@ -458,13 +457,8 @@ extern "C" scrs_status scrs_incr_feed_dev(scrs_incr_encoder *enc, uint64_t index
     const uint64_t L = enc->prof.symbol_len, slot = enc->pending.size();
     hipStream_t cs = static_cast<hipStream_t>(stream);
     hipError_t err = hipMemcpyAsync(enc->d_pending.as<uint8_t>() + slot * L, data_dev, L, hipMemcpyDeviceToDevice, cs);
-    hipEvent_t ev;
-    if (err == hipSuccess) err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (err == hipSuccess) {
-        err = hipEventRecord(ev, cs);
-        if (err == hipSuccess) err = hipStreamWaitEvent(enc->stream, ev, 0);
-        (void)hipEventDestroy(ev);
-    }
+    if (err == hipSuccess) err = hipEventRecord(enc->order_ev, cs);
+    if (err == hipSuccess) err = hipStreamWaitEvent(enc->stream, enc->order_ev, 0);
     if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     enc->fed[index] = 1;
     enc->fed_count++;
@@ -504,13 +498,8 @@ extern "C" scrs_status scrs_incr_repairs_dev(scrs_incr_encoder *enc, uint8_t **r
     if (s.code != SCRS_OK) return s;
     s = flush(enc);
     if (s.code != SCRS_OK) return s;
-    hipEvent_t ev;
-    hipError_t err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (err == hipSuccess) {
-        err = hipEventRecord(ev, enc->stream);
-        if (err == hipSuccess) err = hipStreamWaitEvent(static_cast<hipStream_t>(stream), ev, 0);
-        (void)hipEventDestroy(ev);
-    }
+    hipError_t err = hipEventRecord(enc->order_ev, enc->stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(static_cast<hipStream_t>(stream), enc->order_ev, 0);
     if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     *repairs_dev = enc->d_repairs.as<uint8_t>();
     return ok();

@@ -41,6 +41,10 @@ struct scrs_decode_scratch {
     uint64_t k = 0, m = 0, L = 0;
     scrs::DevBuf d_payload, d_out, d_ptrs;
     scrs::HostBuf h_payload, h_out;
+    // Per-call host state, sized on the first call and reused after.
+    std::vector<uint64_t> seen;
+    std::vector<uint16_t> idx16;
+    std::vector<const uint8_t *> ptrs;
 };
 
 struct scrs_batch_decoder {
@@ -51,6 +55,7 @@ struct scrs_batch_decoder {
     scrs_decode_scratch own{};
     void *afft = nullptr;        // AFFT engine state (afft engine only)
     int32_t kernel = SCRS_KERNEL_TABLE;
+    std::vector<uint64_t> seen;  // index bitmap of the device batch calls, reused
 };
 
 // One cached reconstruction recipe: the device product tables, row count
@@ -84,6 +89,11 @@ struct scrs_decoder {
     std::vector<uint64_t> bits;     // receipt bitmap
     uint64_t distinct = 0, received = 0;
     void *afft = nullptr;
+    // Per-finalize host state (reserved at creation) and the event that
+    // orders device pushes and finalizes against the decoder's stream.
+    std::vector<uint16_t> idx;
+    std::vector<uint64_t> key;
+    hipEvent_t order_ev = nullptr;
 };
 
 struct scrs_incr_encoder {
@@ -104,4 +114,5 @@ struct scrs_incr_encoder {
     uint64_t fed_count = 0;
     bool mirror_valid = false;
     bool feed_ready = false;        // stream and accumulators exist (first feed / repair)
+    hipEvent_t order_ev = nullptr;  // orders device feeds and repairs_dev readers (made with the stream)
 };
