@@ -331,6 +331,40 @@ scrs_status scrs_incr_reset(scrs_incr_encoder *enc);
 scrs_status scrs_incr_encode_batch_dev(const scrs_incr_encoder *enc, const uint8_t *data_dev,
                                        uint8_t *repairs_dev, uint64_t count, void *stream);
 
+/* ---- Additive-FFT transform plan (afft/transform.rs:24-95) --------------- */
+
+typedef struct scrs_transform_plan scrs_transform_plan;
+
+/* TransformPlan::new(size) (afft/transform.rs:41-60): the LCH novel-basis
+ * transform over the additive points 0..size-1 of GF(2^16).  size must be a
+ * power of two in 1..=65536; any other size is SCRS_ERR_INVALID_ARGUMENT
+ * {a = 1} (the reference returns None). */
+scrs_status scrs_transform_plan_create(uint64_t size, scrs_transform_plan **out);
+void scrs_transform_plan_destroy(scrs_transform_plan *plan);
+/* TransformPlan::{size, log_size} (afft/transform.rs:63-71). */
+uint64_t scrs_transform_plan_size(const scrs_transform_plan *plan);
+uint32_t scrs_transform_plan_log_size(const scrs_transform_plan *plan);
+
+/* TransformPlan::forward (afft/transform.rs:73-82): evaluate `len`
+ * novel-basis coefficients at the plan's points, in place.
+ * TransformPlan::inverse (:84-91): evaluations back to coefficients.
+ * len != size is TransformLengthError{a = expected, b = got}.  Host
+ * memory; returns once the result is back. */
+scrs_status scrs_transform_forward(const scrs_transform_plan *plan, uint16_t *values, uint64_t len);
+scrs_status scrs_transform_inverse(const scrs_transform_plan *plan, uint16_t *values, uint64_t len);
+
+/* Device form (no reference counterpart; the byte form the codecs use,
+ * TransformPlan::forward_bytes / inverse_bytes, afft/transform.rs:93-99,
+ * 167-172): `count` blocks back to back, each of `size` rows of symbol_len
+ * bytes (interleaved little-endian elements, symbol_len even), every element
+ * column transformed independently, in place on `stream`.  Element slices
+ * are symbol_len = 2.  An odd symbol_len is SCRS_CONFIG_ODD_SYMBOL_LEN; count
+ * above 65535 is SCRS_ERR_CAPACITY{a = 65535}. */
+scrs_status scrs_transform_forward_dev(const scrs_transform_plan *plan, uint8_t *rows_dev, uint64_t symbol_len,
+                                       uint64_t count, void *stream);
+scrs_status scrs_transform_inverse_dev(const scrs_transform_plan *plan, uint8_t *rows_dev, uint64_t symbol_len,
+                                       uint64_t count, void *stream);
+
 /* ---- Fused batches of uneven codewords (no reference counterpart) ------- */
 
 typedef enum scrs_job_op { SCRS_JOB_ENCODE = 0, SCRS_JOB_DECODE = 1 } scrs_job_op;

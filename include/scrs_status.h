@@ -8,6 +8,7 @@
  *   ConfigError   scrs/src/error.rs:12-31
  *   EncodeError   scrs/src/error.rs:35-71
  *   DecodeError   scrs/src/error.rs:75-126
+ * and the transform's length error (scrs/src/afft/transform.rs:16-22).
  * Codes 90+ have no reference counterpart: they report misuse of the C ABI
  * itself (null handles) or a failure of the HIP runtime.
  */
@@ -47,6 +48,9 @@ typedef enum scrs_code {
     SCRS_DECODE_INSUFFICIENT_RANK = 35,  /* a = rank, b = k */
     SCRS_DECODE_WRONG_OUTPUT_LEN = 36,   /* a = expected, b = got */
     SCRS_DECODE_SCRATCH_MISMATCH = 37,
+
+    /* TransformLengthError (afft/transform.rs:16-22) */
+    SCRS_TRANSFORM_WRONG_LEN = 40,       /* a = expected, b = got */
 
     /* C-ABI / runtime failures (no reference counterpart) */
     SCRS_ERR_INVALID_ARGUMENT = 90, /* null handle or pointer; a = argument position */

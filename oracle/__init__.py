@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
                                                   C.c_size_t, C.c_int]
         L.or_afft_encode_port.restype = C.c_int
         L.or_afft_encode_port.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _u8p, _u8p, C.c_int, C.c_size_t]
+        L.or_afft_novel_eval.argtypes = [_u16p, C.c_size_t, _u16p, C.c_size_t, _u16p]
         L.or_afft_decode_port.restype = C.c_int
         L.or_afft_decode_port.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _szp, C.POINTER(C.c_void_p), _u8p,
                                           C.c_int]
@@ -394,4 +395,14 @@ def afft_decode_port(k: int, m: int, L: int, symbols, threads: int = 1) -> np.nd
     out = np.empty(k * L, np.uint8)
     if lib().or_afft_decode_port(k, m, L, idx, ptrs, _p8(out), threads) != 0:
         raise ValueError("bad geometry")
+    return out
+
+
+def afft_novel_eval(coef: np.ndarray, points) -> np.ndarray:
+    """Direct evaluation of novel-basis coefficients (len a power of two) at
+    the given points: sum_i coef[i] X_i(x)."""
+    c = np.ascontiguousarray(coef, np.uint16)
+    pts = np.ascontiguousarray(points, np.uint16)
+    out = np.empty(pts.size, np.uint16)
+    lib().or_afft_novel_eval(_p16(c), c.size, _p16(pts), pts.size, _p16(out))
     return out

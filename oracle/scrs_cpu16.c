@@ -698,3 +698,25 @@ int or_afft_decode_port(size_t k, size_t m, size_t L, const size_t *idx, const u
     free(f);
     return 0;
 }
+
+/* Direct evaluation in the novel basis (the reference's check,
+ * afft/transform.rs:565-590): out[p] = sum_i coef[i] X_i(points[p]) with
+ * X_i(x) the product of W_b(x) / W_b(2^b) over the set bits b of i. */
+void or_afft_novel_eval(const uint16_t *coef, size_t N, const uint16_t *points, size_t npoints, uint16_t *out) {
+    pthread_once(&tables_once, tables_init);
+    size_t logn = log2z(N);
+    uint16_t c[16][17], ninv[16], wbar[16];
+    subspace_polys(logn, c, ninv);
+    uint16_t *X = (uint16_t *)malloc(N * sizeof(uint16_t));
+    for (size_t p = 0; p < npoints; p++) {
+        for (size_t b = 0; b < logn; b++) wbar[b] = m16(lin_eval(c[b], b + 1, points[p]), ninv[b]);
+        X[0] = 1;
+        uint16_t r = m16(coef[0], 1);
+        for (size_t i = 1; i < N; i++) {
+            X[i] = m16(X[i & (i - 1)], wbar[__builtin_ctzl(i)]);
+            r ^= m16(coef[i], X[i]);
+        }
+        out[p] = r;
+    }
+    free(X);
+}

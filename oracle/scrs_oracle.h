@@ -132,6 +132,7 @@ int or_tower_finalize_port_many(size_t k, size_t m, size_t L, const uint16_t *id
                                 const uint8_t *repairs, uint8_t *out, size_t count, int threads);
 int or_afft_encode_port(size_t k, size_t m, size_t L, const uint8_t *data, uint8_t *repairs, int threads,
                         size_t max_strips);
+void or_afft_novel_eval(const uint16_t *coef, size_t N, const uint16_t *points, size_t npoints, uint16_t *out);
 int or_afft_decode_port(size_t k, size_t m, size_t L, const size_t *idx, const uint8_t *const *payloads,
                         uint8_t *out, int threads);
 

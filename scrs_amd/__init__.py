@@ -21,6 +21,8 @@ from .codec import (  # noqa: F401
     Engine,
     Field,
     Kernel,
+    TransformLengthError,
+    TransformPlan,
     Profile,
     ScrsError,
     batch_decoder,

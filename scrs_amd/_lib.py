@@ -125,6 +125,15 @@ SIGNATURES: dict[str, tuple] = {
     "scrs_incr_fed_count": (C.c_uint64, [H]),
     "scrs_incr_reset": (Status, [H]),
     "scrs_incr_encode_batch_dev": (Status, [H, vp, vp, C.c_uint64, vp]),
+    # additive-FFT transform plan
+    "scrs_transform_plan_create": (Status, [C.c_uint64, HP]),
+    "scrs_transform_plan_destroy": (None, [H]),
+    "scrs_transform_plan_size": (C.c_uint64, [H]),
+    "scrs_transform_plan_log_size": (C.c_uint32, [H]),
+    "scrs_transform_forward": (Status, [H, u16p, C.c_uint64]),
+    "scrs_transform_inverse": (Status, [H, u16p, C.c_uint64]),
+    "scrs_transform_forward_dev": (Status, [H, vp, C.c_uint64, C.c_uint64, vp]),
+    "scrs_transform_inverse_dev": (Status, [H, vp, C.c_uint64, C.c_uint64, vp]),
     # fused uneven batches
     "scrs_batch_engine_create": (Status, [C.POINTER(BatchConfigStruct), HP]),
     "scrs_batch_engine_destroy": (None, [H]),

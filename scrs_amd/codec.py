@@ -80,6 +80,10 @@ class DecodeError(ScrsError):
     """error.rs:75."""
 
 
+class TransformLengthError(ScrsError):
+    """afft/transform.rs:16-22 (a struct there; variant name kept as its own)."""
+
+
 class DeviceError(ScrsError):
     """A HIP runtime failure or C-ABI misuse (no reference counterpart)."""
 
@@ -105,6 +109,7 @@ _CODES = {
     35: (DecodeError, "InsufficientRank", ("rank", "k")),
     36: (DecodeError, "WrongOutputLen", ("expected", "got")),
     37: (DecodeError, "ScratchMismatch", ()),
+    40: (TransformLengthError, "TransformLengthError", ("expected", "got")),
     90: (DeviceError, "InvalidArgument", ("position",)),
     91: (DeviceError, "Device", ("hip_error",)),
     92: (DeviceError, "OutOfMemory", ("bytes",)),
@@ -594,6 +599,59 @@ class IncrementalEncoder(_Coded):
 def incremental_encoder(profile: Profile) -> IncrementalEncoder:
     """incremental_encoder (selector.rs:370-381)."""
     return IncrementalEncoder(profile)
+
+
+# ---------------------------------------------------------------------------
+# Additive-FFT transform plan
+# ---------------------------------------------------------------------------
+
+
+class TransformPlan(_Handle):
+    """TransformPlan (afft/transform.rs:24-95): the novel-basis additive FFT
+    over the points 0..size-1 of GF(2^16), size a power of two up to 65536."""
+
+    _destroy_name = "scrs_transform_plan_destroy"
+
+    def __init__(self, size: int):
+        h = C.c_void_p()
+        check(_lib.load().scrs_transform_plan_create(size, C.byref(h)))
+        super().__init__(h.value)
+
+    @staticmethod
+    def new(size: int) -> "TransformPlan | None":
+        """TransformPlan::new: None for a size that is not a power of two in 1..=65536."""
+        if size <= 0 or size > 65536 or size & (size - 1):
+            return None
+        return TransformPlan(size)
+
+    @property
+    def size(self) -> int:
+        return int(_lib.load().scrs_transform_plan_size(self._h))
+
+    @property
+    def log_size(self) -> int:
+        return int(_lib.load().scrs_transform_plan_log_size(self._h))
+
+    def _host(self, fn, values: np.ndarray) -> None:
+        if not (isinstance(values, np.ndarray) and values.dtype == np.uint16 and values.flags.c_contiguous):
+            raise TypeError("values must be a contiguous uint16 array (modified in place)")
+        check(fn(self._h, values.ctypes.data_as(_lib.u16p), values.size))
+
+    def forward(self, values: np.ndarray) -> None:
+        """Evaluate novel-basis coefficients at the points, in place."""
+        self._host(_lib.load().scrs_transform_forward, values)
+
+    def inverse(self, values: np.ndarray) -> None:
+        """Evaluations back to novel-basis coefficients, in place."""
+        self._host(_lib.load().scrs_transform_inverse, values)
+
+    def forward_dev(self, rows_dev, symbol_len: int, count: int = 1, stream=None) -> None:
+        check(_lib.load().scrs_transform_forward_dev(self._h, _dev_ptr(rows_dev), symbol_len, count,
+                                                     _stream_ptr(stream)))
+
+    def inverse_dev(self, rows_dev, symbol_len: int, count: int = 1, stream=None) -> None:
+        check(_lib.load().scrs_transform_inverse_dev(self._h, _dev_ptr(rows_dev), symbol_len, count,
+                                                     _stream_ptr(stream)))
 
 
 # ---------------------------------------------------------------------------

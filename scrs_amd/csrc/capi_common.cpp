@@ -48,6 +48,7 @@ extern "C" const char *scrs_status_name(int32_t code) {
         case SCRS_DECODE_INSUFFICIENT_RANK: return "DecodeError::InsufficientRank";
         case SCRS_DECODE_WRONG_OUTPUT_LEN: return "DecodeError::WrongOutputLen";
         case SCRS_DECODE_SCRATCH_MISMATCH: return "DecodeError::ScratchMismatch";
+        case SCRS_TRANSFORM_WRONG_LEN: return "TransformLengthError";
         case SCRS_ERR_INVALID_ARGUMENT: return "InvalidArgument";
         case SCRS_ERR_DEVICE: return "DeviceError";
         case SCRS_ERR_OUT_OF_MEMORY: return "OutOfMemory";

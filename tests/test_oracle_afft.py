@@ -193,3 +193,23 @@ def test_proptest_regressions_round_trip(case):
     trep = oracle.tower_encode(c, k, m, L, data)
     tsym = [data[i * L:(i + 1) * L] for i in range(k)] + [trep[j * L:(j + 1) * L] for j in range(m)]
     assert np.array_equal(oracle.tower_decode_rref(c, k, m, L, [(i, tsym[i]) for i in idx]), data)
+
+
+def test_direct_novel_evaluation_matches_the_basis_expansion():
+    """oracle.afft_novel_eval (the checker of the device transform) against
+    the products of normalised subspace polynomials computed here."""
+    for n in (1, 2, 4, 8, 32):
+        rng = np.random.default_rng(n)
+        coef = rng.integers(0, 65536, n, dtype=np.uint16)
+        points = np.arange(min(2 * n, 64), dtype=np.uint16)
+        got = oracle.afft_novel_eval(coef, points)
+        for p, x in enumerate(points):
+            nb = novel_basis(int(x), 16)
+            want = 0
+            for i in range(n):
+                xi = 1
+                for b in range(n.bit_length() - 1):
+                    if (i >> b) & 1:
+                        xi = mul(xi, nb[b])
+                want ^= mul(int(coef[i]), xi)
+            assert want == got[p], (n, int(x))
