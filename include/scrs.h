@@ -76,14 +76,15 @@ uint64_t scrs_device_alloc_count(void);
 uint64_t scrs_host_alloc_count(void);
 
 /* Term split of small codec launches (no reference counterpart).  When
- * enabled, a codec launch too small to fill the device (a lone codeword,
- * say) splits each codeword's terms across workgroups that XOR their
- * partial rows into destination rows cleared first; it applies to launches
- * whose destination rows and symbol_len are multiples of 4 bytes and that
- * overwrite their destinations.  The bytes are the same either way.  Off by
- * default: the codec entry points then run the plain per-codeword path, one
- * workgroup per row group and column tile folding every term.
- * Process-wide; returns the previous setting. */
+ * enabled (the default), a codec launch too small to fill the device (a lone
+ * codeword, say) splits each codeword's terms across workgroups: each
+ * segment stores its partial rows to a buffer the codec keeps, and a second
+ * kernel XORs them into the destinations (atomic XORs into rows cleared
+ * first when no buffer fits).  It applies to launches whose destination
+ * rows and symbol_len are multiples of 4 bytes and that overwrite their
+ * destinations.  The bytes are the same either way.  Disabled, a lone
+ * codeword runs on one workgroup per row group and column tile, each
+ * folding every term.  Process-wide; returns the previous setting. */
 int32_t scrs_set_term_split(int32_t enable);
 
 /* Largest number of missing data symbols for which an additive-FFT decode

@@ -257,7 +257,11 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
         a.tables = enc->d_bm_tables.as<uint32_t>();
         err = launch_rows_bitmatrix(a, static_cast<hipStream_t>(stream));
     } else {
-        err = launch_rows_terms_lds(a, vec16, static_cast<hipStream_t>(stream));
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        auto *mut = const_cast<scrs_batch_encoder *>(enc);
+        const bool attached = mut->split.acquire(a, static_cast<uint32_t>(m), s, true);
+        err = launch_rows_terms_lds(a, vec16, s);
+        if (attached) mut->split.release(s);
     }
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }

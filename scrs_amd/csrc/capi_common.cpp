@@ -9,7 +9,7 @@ extern "C" uint64_t scrs_device_alloc_count(void) { return scrs::alloc_counter()
 
 namespace scrs {
 namespace {
-std::atomic<int32_t> g_term_split{0};
+std::atomic<int32_t> g_term_split{1};
 std::atomic<int32_t> g_afft_targeted_max{static_cast<int32_t>(kAfftTargetedDefault)};
 }  // namespace
 bool term_split_enabled() { return g_term_split.load(std::memory_order_relaxed) != 0; }

@@ -101,5 +101,8 @@ extern "C" scrs_status scrs_decode_batch_dev_noalloc(scrs_batch_decoder *dec, co
     const bool vec16 = L % 16 == 0 && (reinterpret_cast<uintptr_t>(out_dev) & 15u) == 0 &&
                        (reinterpret_cast<uintptr_t>(data_dev) & 15u) == 0 &&
                        (reinterpret_cast<uintptr_t>(repairs_dev) & 15u) == 0;
-    return dec->eng.run_chunk(ws, nullptr, data_dev, repairs_dev, out_dev, count, s, vec16, false);
+    // A split launch reuses the decoder's partial-row buffer only if it
+    // already holds it, else XORs atomically.
+    return dec->eng.run_chunk(ws, nullptr, data_dev, repairs_dev, out_dev, count, s, vec16, false,
+                              /*may_allocate=*/false);
 }

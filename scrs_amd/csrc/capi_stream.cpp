@@ -544,6 +544,10 @@ extern "C" scrs_status scrs_incr_encode_batch_dev(const scrs_incr_encoder *enc, 
     a.L = L;
     a.count = count;
     a.dst_aligned4 = (reinterpret_cast<uintptr_t>(repairs_dev) & 3u) == 0 && L % 4 == 0;
-    hipError_t err = launch_rows_terms_field(enc->gf16, a, vec16, static_cast<hipStream_t>(stream));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto *mut = const_cast<scrs_incr_encoder *>(enc);
+    const bool attached = mut->split.acquire(a, static_cast<uint32_t>(m), s, true);
+    hipError_t err = launch_rows_terms_field(enc->gf16, a, vec16, s);
+    if (attached) mut->split.release(s);
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }

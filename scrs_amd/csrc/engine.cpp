@@ -63,6 +63,37 @@ hipError_t launch_rows_terms_field(bool gf16, const RowsTermsArgs &a, bool vec16
     return gf16 ? launch_rows_terms16(a, vec16, s) : launch_rows_terms_lds(a, vec16, s);
 }
 
+SplitBuffer::~SplitBuffer() {
+    if (done) (void)hipEventDestroy(done);
+}
+
+bool SplitBuffer::acquire(RowsTermsArgs &a, uint32_t rows, hipStream_t s, bool may_allocate) {
+    if (rows == 0 || rows > a.rmax || !a.dst_aligned4 || a.L % 4) return false;
+    const uint32_t segs = rows_terms_segments(a);
+    if (segs < 2) return false;
+    const uint64_t need = a.count * rows * segs * a.L;
+    if (need > kMaxBytes) return false;
+    mu.lock();
+    hipError_t err = hipSuccess;
+    if (need > buf.size) err = may_allocate ? buf.ensure(need) : hipErrorOutOfMemory;
+    if (err == hipSuccess && !done) err = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s, done, 0);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();  // no buffer only means atomics
+        mu.unlock();
+        return false;
+    }
+    a.partials = buf.as<uint8_t>();
+    a.partial_rows = rows;
+    a.partials_bytes = need;
+    return true;
+}
+
+void SplitBuffer::release(hipStream_t s) {
+    (void)hipEventRecord(done, s);
+    mu.unlock();
+}
+
 DecodeEngine::~DecodeEngine() {
     if (idx_copied) (void)hipEventDestroy(idx_copied);
 }
@@ -208,7 +239,7 @@ DecodeWorkspace DecodeEngine::carve_workspace(uint8_t *base, uint64_t count) con
 
 scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *const *payload_ptrs,
                                     const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out, uint64_t cn,
-                                    hipStream_t s, bool vec16, bool bitmatrix) {
+                                    hipStream_t s, bool vec16, bool bitmatrix, bool may_allocate) {
     const uint64_t tb = cw_table_bytes(bitmatrix);
     // GF(2^8) table path: the multiply-XOR builds its product tables in LDS
     // from the fused coefficients, so none are written to HBM.
@@ -275,18 +306,9 @@ scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *co
     ra.force_split = force_split;
     // Missing rows are out + index * L.
     ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
-    if (force_split && gf16 && rows_hint && rows_hint <= rmax && k >= 2 * kSplitMinTerms) {
-        // Sized for the segments this launch actually plans.
-        const uint64_t need = cn * rows_hint * rows_terms16_segments(ra) * L;
-        if (need <= (1ull << 30) && d_partials.ensure(need) == hipSuccess) {
-            ra.partials = d_partials.as<uint8_t>();
-            ra.partial_rows = rows_hint;
-            ra.partials_bytes = need;
-        } else {
-            (void)hipGetLastError();  // a failed allocation only means atomics
-        }
-    }
+    const bool attached = !bitmatrix && split.acquire(ra, force_split && rows_hint ? rows_hint : rmax, s, may_allocate);
     err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);
+    if (attached) split.release(s);
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }
 
@@ -336,7 +358,9 @@ scrs_status DecodeEngine::enqueue_recipe(const uint16_t *indices, const uint32_t
     ra.L = L;
     ra.count = 1;
     ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
+    const bool attached = split.acquire(ra, rmax, s, true);
     err = launch_rows_terms_field(gf16, ra, vec16, s);
+    if (attached) split.release(s);
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "../../include/scrs.h"
@@ -34,6 +35,30 @@ std::vector<uint32_t> coefficient_tables(const scrs_profile &p, const std::vecto
 
 // Launch the multiply-XOR of the profile's field.
 hipError_t launch_rows_terms_field(bool gf16, const RowsTermsArgs &a, bool vec16, hipStream_t s);
+
+// Partial-row buffer of a codec's term-split launches.  A launch too small
+// to fill the device splits each codeword's terms into segments; each
+// segment stores its partial rows here and a reduce kernel XORs them into
+// the destinations.  One launch holds the buffer at a time (the lock), and a
+// launch on another stream first waits for the previous one's work.
+struct SplitBuffer {
+    std::mutex mu;
+    DevBuf buf;
+    hipEvent_t done = nullptr;
+    static constexpr uint64_t kMaxBytes = 256ull << 20;
+
+    SplitBuffer() = default;
+    SplitBuffer(const SplitBuffer &) = delete;
+    SplitBuffer &operator=(const SplitBuffer &) = delete;
+    ~SplitBuffer();
+    // When `a`'s launch splits, give it partial rows for rows < `rows` (an
+    // upper bound of every codeword's row count): takes the lock, sizes the
+    // buffer (only if may_allocate) and orders `s` after the buffer's previous
+    // launch.  True when attached; the caller launches and then calls
+    // release(s).  False leaves `a` to the launcher's atomic split.
+    bool acquire(RowsTermsArgs &a, uint32_t rows, hipStream_t s, bool may_allocate);
+    void release(hipStream_t s);
+};
 
 // Device workspace of one decode launch: received indices, the solve's
 // outputs and work, and the pointer tables, for `cap` codewords.
@@ -71,10 +96,10 @@ struct DecodeEngine {
     DevBuf d_wlog;
     bool weighted = false, force_split = false;
     // Forced splits: the caller's exact row count of the next enqueue (0 =
-    // unknown); with it the split segments store partial rows to d_partials
-    // and a second kernel combines them, instead of atomic XORs.
+    // unknown), which bounds the partial rows; other launches bound them by
+    // rmax.
     uint32_t rows_hint = 0;
-    DevBuf d_partials;
+    SplitBuffer split;
 
     DecodeEngine() = default;
     DecodeEngine(const DecodeEngine &) = delete;
@@ -103,9 +128,11 @@ struct DecodeEngine {
     // The chain of one launch chunk of `cn` codewords whose indices are
     // already in ws.idx (payload, data, repair and output bases of the
     // chunk's first codeword).
+    // may_allocate = false: a split launch uses the partial-row buffer only
+    // if it already holds the launch, else atomics.
     scrs_status run_chunk(const DecodeWorkspace &ws, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
                           const uint8_t *repair_base, uint8_t *out, uint64_t cn, hipStream_t s, bool vec16,
-                          bool bitmatrix);
+                          bool bitmatrix, bool may_allocate = true);
     // Bytes a caller-provided workspace needs for `count` codewords of the
     // GF(2^8) table kernel (no product tables in HBM): each array starts on
     // a 256-byte boundary.  carve_workspace lays them out from `base`.

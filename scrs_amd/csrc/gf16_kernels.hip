@@ -258,13 +258,17 @@ __global__ __launch_bounds__(kThreads) void rows_terms16_kernel(RowsTermsArgs a,
 
 }  // namespace
 
-uint32_t rows_terms16_segments(const RowsTermsArgs &args) {
-    RowsTermsArgs a = args;
-    if (a.count == 0 || a.L == 0) return 1;
-    const uint64_t tiles = (a.L + kTile - 1) / kTile;
-    const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
-    plan_term_split(a, a.count * groups * tiles);
-    return a.segs;
+hipError_t launch_reduce_partials(const RowsTermsArgs &a, hipStream_t s) {
+    const uint64_t words = a.L / 4;
+    if (a.segs >= 256 && a.count * a.partial_rows * words < 32768) {
+        const dim3 rgrid(static_cast<uint32_t>((words + kReduceDwords - 1) / kReduceDwords), a.partial_rows,
+                         static_cast<uint32_t>(a.count));
+        hipLaunchKernelGGL(reduce_partials_wide_kernel, rgrid, dim3(256), 0, s, a);
+    } else {
+        const dim3 rgrid(static_cast<uint32_t>((words + 255) / 256), a.partial_rows, static_cast<uint32_t>(a.count));
+        hipLaunchKernelGGL(reduce_partials_kernel, rgrid, dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
@@ -289,10 +293,7 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
     const uint32_t t32 = static_cast<uint32_t>(tiles), u32 = static_cast<uint32_t>(units);
     // Partial rows when the caller's buffer holds them (rows bounded by
     // partial_rows, L a multiple of 4 as every split requires).
-    if (!split || a.partial_rows == 0 || a.partial_rows > a.rmax ||
-        a.count * a.partial_rows * a.segs * a.L > a.partials_bytes || (a.L / 4 + 3) / 4 > 0x7fffffffull ||
-        a.count > 65535)
-        a.partials = nullptr;
+    if (!split || !partials_usable(a)) a.partials = nullptr;
     if (split && !a.partials) {
         hipError_t err = launch_zero_dst_rows(a, s);
         if (err != hipSuccess) return err;
@@ -320,16 +321,7 @@ hipError_t launch_rows_terms16(const RowsTermsArgs &args, bool vec16, hipStream_
         hipLaunchKernelGGL((rows_terms16_kernel<false, false, 0>), grid, dim3(kThreads), 0, s, a, t32, u32);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || !a.partials) return err;
-    const uint64_t words = a.L / 4;
-    if (a.segs >= 256 && a.count * a.partial_rows * words < 32768) {
-        const dim3 rgrid(static_cast<uint32_t>((words + kReduceDwords - 1) / kReduceDwords), a.partial_rows,
-                         static_cast<uint32_t>(a.count));
-        hipLaunchKernelGGL(reduce_partials_wide_kernel, rgrid, dim3(256), 0, s, a);
-    } else {
-        const dim3 rgrid(static_cast<uint32_t>((words + 255) / 256), a.partial_rows, static_cast<uint32_t>(a.count));
-        hipLaunchKernelGGL(reduce_partials_kernel, rgrid, dim3(256), 0, s, a);
-    }
-    return hipGetLastError();
+    return launch_reduce_partials(a, s);
 }
 
 }  // namespace scrs

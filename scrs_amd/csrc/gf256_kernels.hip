@@ -193,6 +193,12 @@ __device__ void rows_terms_pass(const RowsTermsArgs &a, uint8_t *lds, uint64_t c
             o[j] = ((acc[4 * j + 0][d] >> s) & 0xffu) | (((acc[4 * j + 1][d] >> s) & 0xffu) << 8) |
                    (((acc[4 * j + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * j + 3][d] >> s) & 0xffu) << 24);
         }
+        if (a.segs > 1 && a.partials) {  // a term segment: its own partial row
+            if (static_cast<uint32_t>(row) >= a.partial_rows) continue;  // outside the buffer's rows
+            const uint64_t slot = (c * a.partial_rows + row) * a.segs + tb / a.seg_terms;
+            store16<VEC>(a.partials + slot * a.L, p, a.L, o);
+            continue;
+        }
         if (a.segs > 1) {  // a term segment: XOR into the cleared row
             xor16_atomic(dst, p, a.L, o);
             continue;
@@ -314,6 +320,21 @@ hipError_t launch_zero_dst_rows(const RowsTermsArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+uint32_t rows_terms_segments(const RowsTermsArgs &args) {
+    RowsTermsArgs a = args;
+    if (a.count == 0 || a.L == 0) return 1;
+    const uint64_t tiles = (a.L + kTile - 1) / kTile;
+    const uint32_t groups = a.rmax ? (a.rmax + kRowsPerPass - 1) / kRowsPerPass : 1;
+    plan_term_split(a, a.count * groups * tiles);
+    return a.segs;
+}
+
+bool partials_usable(const RowsTermsArgs &a) {
+    return a.segs > 1 && a.partials && a.partial_rows != 0 && a.partial_rows <= a.rmax &&
+           a.count * a.partial_rows * a.segs * a.L <= a.partials_bytes && (a.L / 4 + 255) / 256 <= 0x7fffffffull &&
+           a.count <= 65535;
+}
+
 hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStream_t s) {
     // The 16-byte paths address a row with a 32-bit column offset.
     if (args.L > 0xffffffffull) vec16 = false;
@@ -334,7 +355,10 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStrea
     if (dec != (a.dst_ptrs != nullptr) || dec != (a.copy_ptrs != nullptr) || dec != (a.rows != nullptr))
         return hipErrorInvalidValue;
     const bool full = vec16 && a.T % kTermChunk == 0 && !split;
-    if (split) {
+    // Partial rows and a reduce when the caller's buffer holds them, else
+    // atomic XORs into rows cleared first.
+    if (!partials_usable(a)) a.partials = nullptr;
+    if (split && !a.partials) {
         hipError_t err = launch_zero_dst_rows(a, s);
         if (err != hipSuccess) return err;
     }
@@ -350,7 +374,9 @@ hipError_t launch_rows_terms_lds(const RowsTermsArgs &args, bool vec16, hipStrea
         hipLaunchKernelGGL((rows_terms_lds_kernel<false, true, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
     else
         hipLaunchKernelGGL((rows_terms_lds_kernel<false, false, false>), grid, dim3(kThreads), 0, s, a, t32, u32);
-    return hipGetLastError();
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || !a.partials) return err;
+    return launch_reduce_partials(a, s);
 }
 
 hipError_t launch_fill_splitmix(uint8_t *dst, uint64_t len, uint64_t seed, uint64_t byte_offset, hipStream_t s) {

@@ -27,6 +27,7 @@ struct scrs_batch_encoder {
     scrs::DevBuf d_tables;       // GF(2^8) product tables of all k terms
     int32_t kernel = SCRS_KERNEL_TABLE;
     scrs::DevBuf d_bm_tables;    // bit-matrix operands (SCRS_KERNEL_BITMATRIX)
+    scrs::SplitBuffer split;     // partial rows of launches too small to fill the device
     hipStream_t stream = nullptr;  // host-slice calls only, created on first use
     std::once_flag stream_once;
     hipError_t stream_err = hipSuccess;
@@ -109,6 +110,7 @@ struct scrs_incr_encoder {
     scrs::DevBuf d_ptab;            // their gathered tables
     scrs::HostBuf h_pending;        // pinned staging of host feeds
     scrs::HostBuf h_repairs;        // host mirror returned by repair()
+    scrs::SplitBuffer split;        // partial rows of lone-codeword batch encodes
     std::vector<uint32_t> pending;  // data indices staged, in feed order
     std::vector<uint8_t> fed;
     uint64_t fed_count = 0;

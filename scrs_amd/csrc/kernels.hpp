@@ -72,8 +72,18 @@ constexpr uint64_t kSplitFillBlocks = 512;
 #endif
 constexpr uint64_t kForceSplitFillBlocks = SCRS_FORCE_SPLIT_BLOCKS;
 bool plan_term_split(RowsTermsArgs &a, uint64_t active_blocks);
-// The process-wide switch of the split (scrs_set_term_split), off by default.
+// The process-wide switch of the split (scrs_set_term_split), on by default.
 bool term_split_enabled();
+// Term segments per codeword that either field's multiply-XOR launcher plans
+// for `a` (1 when it does not split): sizes a split launch's partial rows,
+// count * partial_rows * segments * L bytes.  Both fields tile 4096 columns
+// and 16 rows per workgroup.
+uint32_t rows_terms_segments(const RowsTermsArgs &a);
+// Whether `a` (split planned) can store partial rows to its buffer.
+bool partials_usable(const RowsTermsArgs &a);
+// Second kernel of a split launch with partial rows: each destination row
+// is the XOR of its segments' partial rows.
+hipError_t launch_reduce_partials(const RowsTermsArgs &a, hipStream_t s);
 // Clears the destination rows of a split launch (a kernel, no memset).
 hipError_t launch_zero_dst_rows(const RowsTermsArgs &a, hipStream_t s);
 
@@ -139,9 +149,6 @@ hipError_t launch_bitmatrix_tables(const DecodePrepArgs &a, hipStream_t s);
 // addressing as RowsTermsArgs; tables in the 2048 B per (term, group) layout
 // (build_nibble_tables16).
 hipError_t launch_rows_terms16(const RowsTermsArgs &a, bool vec16, hipStream_t s);
-// Term segments per codeword that launch_rows_terms16 plans for `a` (1 when
-// it does not split): sizes the partial-row buffer of a split launch.
-uint32_t rows_terms16_segments(const RowsTermsArgs &a);
 
 // Bit-matrix path: the same multiply-XOR as a GF(2) product on the int8
 // matrix cores.  Tables hold, per codeword, row group g and 8-term chunk ch,
