@@ -636,7 +636,7 @@ def serve_workload(args, bounded: bool = False) -> dict:
     setup.clear(1)
     plain_sat = run(plain, 1, None, "plain")
     verify(1, "plain")
-    batched = Server(batched=True, max_batch_jobs=args.max_batch, tile_bytes=0, cw_per_unit=0, use_graph=True,
+    batched = Server(batched=True, max_batch_jobs=args.max_batch, tile_bytes=0, cw_per_unit=0, use_graph=False,
                      cache_bytes=16 << 30)
     run(batched, 0, None, "batched warm-up")
     setup.clear(0)

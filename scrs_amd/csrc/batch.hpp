@@ -66,11 +66,12 @@ struct BatchHeader {
 constexpr uint32_t kBatchTileBytes[3] = {1024, 2048, 4096};
 
 // The launches read every size and array address from the device header, so
-// their arguments stay fixed across batches.  The solve launch also clears
-// the unit counter of the multiply-XOR launch that follows it (a kernel
-// write, so a captured chain holds no memset node).
+// their arguments stay fixed across batches.  The multiply-XOR launch takes
+// units from counter[0] and leaves counter[0..1] at zero when it ends (zero
+// before the first launch), so no memset sits between launches and a batch
+// with nothing to solve runs the multiply-XOR alone.
 hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, const uint16_t *gf16_log_exp,
-                              uint32_t *counter, uint32_t grid, hipStream_t s);
+                              uint32_t grid, hipStream_t s);
 hipError_t launch_batch_rows(const BatchHeader *hdr, uint32_t *counter, uint32_t tile_bytes, uint32_t grid,
                              hipStream_t s);
 // Persistent grid that fills the device for the given tile width.

@@ -80,7 +80,13 @@ BatchEngine::~BatchEngine() {
         for (auto &kv : s.graphs) (void)hipGraphExecDestroy(kv.second.second);
     }
     if (capture_) (void)hipStreamDestroy(capture_);
-    if (last_done_) (void)hipEventDestroy(last_done_);
+    if (last_solved_) (void)hipEventDestroy(last_solved_);
+    for (Sample &x : samples_) {
+        (void)hipEventSynchronize(x.e1);
+        event_pool_.push_back(x.e0);
+        event_pool_.push_back(x.e1);
+    }
+    for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     for (CacheEntry *e : lru_) delete e;
     for (CacheEntry *e : retired_) delete e;
 }
@@ -109,10 +115,11 @@ scrs_status BatchEngine::init(const scrs_batch_config &cfg) {
         err = hipMemcpy(gf8_.ptr, t, 768, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = gf16_log_exp_device(&gf16_log_exp_);
     if (err == hipSuccess) err = arena_.init(cfg_.cache_bytes);
-    if (err == hipSuccess) err = hipEventCreateWithFlags(&last_done_, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&last_solved_, hipEventDisableTiming);
     for (Slot &s : slots_) {
         if (err == hipSuccess) err = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         if (err == hipSuccess) err = s.counter.ensure(64);
+        if (err == hipSuccess) err = hipMemset(s.counter.ptr, 0, 64);  // the kernel keeps it at zero after
     }
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }
@@ -184,8 +191,17 @@ void BatchEngine::touch(CacheEntry *e) {
 }
 
 void BatchEngine::retire_completed() {
-    for (Slot &s : slots_)
-        if (s.seq && hipEventQuery(s.done) == hipSuccess) completed_ = std::max(completed_, s.seq);
+    // Batches may finish out of order on several streams: every batch up to
+    // the oldest one still running is complete (a slot is reused only once
+    // its previous batch has finished, so older batches are done).
+    uint64_t oldest_running = 0, newest = 0;
+    for (Slot &s : slots_) {
+        if (!s.seq) continue;
+        newest = std::max(newest, s.seq);
+        if (hipEventQuery(s.done) != hipSuccess && (!oldest_running || s.seq < oldest_running))
+            oldest_running = s.seq;
+    }
+    completed_ = std::max(completed_, oldest_running ? oldest_running - 1 : newest);
     auto it = std::remove_if(retired_.begin(), retired_.end(), [&](CacheEntry *e) {
         if (e->last_use > completed_) return false;
         arena_.release(e->off, e->bytes);
@@ -489,12 +505,10 @@ hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
         hipGraphExec_t exec = nullptr;
         hipError_t err = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal);
         if (err != hipSuccess) return err;
-        // The queue counter is cleared by the solve kernel: no memset nodes.
-        hipError_t e2 = launch_batch_solve(hdr, gf8_.as<uint8_t>(), gf16_log_exp_, slot.counter.as<uint32_t>(),
-                                           kSolveGrid, capture_);
+        // The kernel returns its unit counter to zero itself: no memset node.
         hipError_t e3 = launch_batch_rows(hdr, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), capture_);
         err = hipStreamEndCapture(capture_, &graph);
-        if (err == hipSuccess) err = e2 != hipSuccess ? e2 : e3;
+        if (err == hipSuccess) err = e3;
         if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (graph) (void)hipGraphDestroy(graph);
         if (err != hipSuccess) return err;
@@ -561,22 +575,23 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     if (!p.ptrs.empty()) std::memcpy(h + off_p, p.ptrs.data(), p.ptrs.size() * 8);
     if (!p.u16s.empty()) std::memcpy(h + off_u, p.u16s.data(), p.u16s.size() * 2);
     if (!p.zero.empty()) std::memcpy(h + off_z, p.zero.data(), p.zero.size() * 8);
-    // Tables and recipes written by the previous batch may be read here, and
-    // it may have run on another stream.
-    if ((err = hipStreamWaitEvent(s, last_done_, 0)) != hipSuccess) return err;
+    // Tables and recipes written by the previous batch's solve may be read
+    // here, and it may have run on another stream.  Only its solve is
+    // waited for: its multiply-XOR may still run beside this batch.
+    if ((err = hipStreamWaitEvent(s, last_solved_, 0)) != hipSuccess) return err;
     if ((err = hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s)) != hipSuccess) return err;
-    if (cfg_.use_graph) {
-        err = run_graph(slot, tile, s);
-    } else {
-        const BatchHeader *dh = slot.dev.as<BatchHeader>();
-        if ((err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, slot.counter.as<uint32_t>(), kSolveGrid,
-                                      s)) == hipSuccess)
-            err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), s);
-    }
+    const BatchHeader *dh = slot.dev.as<BatchHeader>();
+    // Nothing to solve and no split destinations to clear (every table and
+    // recipe resident): the multiply-XOR runs alone.
+    if ((!p.solves.empty() || !p.zero.empty()) &&
+        (err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, s)) != hipSuccess)
+        return err;
+    if ((err = hipEventRecord(last_solved_, s)) != hipSuccess) return err;
+    err = cfg_.use_graph ? run_graph(slot, tile, s)
+                         : launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), s);
     if (err != hipSuccess) return err;
     slot.seq = seq_;
-    if ((err = hipEventRecord(slot.done, s)) != hipSuccess) return err;
-    return hipEventRecord(last_done_, s);
+    return hipEventRecord(slot.done, s);
 }
 
 scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_t s, uint64_t *failed) {
@@ -607,78 +622,107 @@ scrs_status BatchEngine::submit(const scrs_job *jobs, uint64_t count, hipStream_
     std::sort(sig_keys_.begin(), sig_keys_.end());
     const uint32_t max_rep =
         std::adjacent_find(sig_keys_.begin(), sig_keys_.end()) != sig_keys_.end() ? 2u : (enc ? 1u : 0u);
-    const std::vector<uint32_t> sig = {log2_floor(lsum / count), gf16 ? 1u : 0u,
-                                       static_cast<uint32_t>(4 * enc / (count + 1)), max_rep > 1 ? 1u : 0u};
+    const Sig sig = {log2_floor(lsum / count), gf16 ? 1u : 0u, static_cast<uint32_t>(4 * enc / (count + 1)),
+                     max_rep > 1 ? 1u : 0u};
     uint32_t tile = cfg_.tile_bytes, cw = cfg_.cw_per_unit;
     const bool tune = (tile == 0 || cw == 0);
+    harvest_samples();
     auto found = tuned_.find(sig);
     if (tune && found != tuned_.end()) {
         if (!tile) tile = found->second.first;
         if (!cw) cw = found->second.second;
     }
-    Plan p;
-    scrs_status r;
-    // Small batches are not representative: they run with the defaults
-    // until a large batch of their signature has been tuned.
+    // A signature not tuned yet runs each large batch on its next candidate
+    // configuration and times it on the device; small batches are not
+    // representative and run with the defaults until then.  Nothing waits:
+    // the timings are collected by later submissions.
+    Sample sample;
+    bool timed = false;
     if (tune && found == tuned_.end() && count >= kTuneMinJobs) {
-        // Time every candidate on this batch (each run is a full, correct
-        // execution; the first, untimed, fills the resident caches).
-        std::vector<uint32_t> tiles = tile ? std::vector<uint32_t>{tile} : std::vector<uint32_t>{1024, 2048, 4096};
-        std::vector<uint32_t> cws = cw ? std::vector<uint32_t>{cw}
-                                       : (max_rep > 1 ? std::vector<uint32_t>{1, 2, 4, 8} : std::vector<uint32_t>{1});
-        hipEvent_t e0, e1;
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
-        float best = -1;
-        uint32_t bt = tiles[0], bc = cws[0];
-        bool warm = false;
-        for (uint32_t tt : tiles)
-            for (uint32_t cc : cws) {
-                for (int rep_i = warm ? 0 : -1; rep_i < 1; rep_i++) {
-                    seq_++;
-                    r = plan(jobs, count, tt, cc, p);
-                    if (r.code != SCRS_OK) return r;
-                    Slot &slot = slots_[next_slot_];
-                    next_slot_ ^= 1u;
-                    (void)hipEventRecord(e0, s);
-                    hipError_t err = launch(slot, p, tt, s);
-                    if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
-                    (void)hipEventRecord(e1, s);
-                    if ((err = hipEventSynchronize(e1)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
-                    float ms = 0;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    if (rep_i == 0 && (best < 0 || ms < best)) {
-                        best = ms;
-                        bt = tt;
-                        bc = cc;
-                    }
-                }
-                warm = true;
+        Tuning &tg = tuning_[sig];
+        if (tg.cands.empty()) {
+            const std::vector<uint32_t> tiles =
+                cfg_.tile_bytes ? std::vector<uint32_t>{cfg_.tile_bytes} : std::vector<uint32_t>{1024, 2048, 4096};
+            const std::vector<uint32_t> cws = cfg_.cw_per_unit ? std::vector<uint32_t>{cfg_.cw_per_unit}
+                                              : (max_rep > 1 ? std::vector<uint32_t>{1, 2, 4, 8}
+                                                             : std::vector<uint32_t>{1});
+            for (uint32_t tt : tiles)
+                for (uint32_t cc : cws) tg.cands.push_back({tt, cc, -1.0});
+        }
+        const uint32_t ci = tg.next % static_cast<uint32_t>(tg.cands.size());
+        tg.next++;
+        tile = tg.cands[ci].tile;
+        cw = tg.cands[ci].cw;
+        sample.sig = sig;
+        sample.cand = ci;
+        for (uint64_t i = 0; i < count; i++) {
+            const scrs_job &j = jobs[i];
+            const uint64_t k = j.profile.k, m = j.profile.m, L = j.profile.symbol_len;
+            sample.bytes += j.op == SCRS_JOB_ENCODE ? (k + m) * L : 2 * k * L;
+        }
+        for (hipEvent_t *e : {&sample.e0, &sample.e1}) {
+            if (!event_pool_.empty()) {
+                *e = event_pool_.back();
+                event_pool_.pop_back();
+            } else if (hipEventCreate(e) != hipSuccess) {
+                *e = nullptr;
             }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        tuned_[sig] = {bt, bc};
-        st_.tuned_signatures = static_cast<uint32_t>(tuned_.size());
-        st_.tile_bytes = bt;
-        st_.cw_per_unit = bc;
-        st_.batches++;
-        st_.jobs += count;
-        return ok();
+        }
+        timed = sample.e0 && sample.e1 && hipEventRecord(sample.e0, s) == hipSuccess;
     }
     if (!tile) tile = 4096;
     if (!cw) cw = 1;
     seq_++;
-    r = plan(jobs, count, tile, cw, p);
+    Plan p;
+    scrs_status r = plan(jobs, count, tile, cw, p);
     if (r.code != SCRS_OK) return r;
     Slot &slot = slots_[next_slot_];
-    next_slot_ ^= 1u;
+    next_slot_ = (next_slot_ + 1) % kSlots;
     hipError_t err = launch(slot, p, tile, s);
     if (err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    if (timed && hipEventRecord(sample.e1, s) == hipSuccess) {
+        samples_.push_back(std::move(sample));
+    } else {
+        if (sample.e0) event_pool_.push_back(sample.e0);
+        if (sample.e1) event_pool_.push_back(sample.e1);
+    }
     st_.tile_bytes = tile;
     st_.cw_per_unit = cw;
     st_.batches++;
     st_.jobs += count;
     return ok();
+}
+
+// Timings of finished tuning runs; a signature whose candidates have all
+// run keeps the one with the least device time per byte.
+void BatchEngine::harvest_samples() {
+    while (!samples_.empty() && hipEventQuery(samples_.front().e1) == hipSuccess) {
+        Sample x = std::move(samples_.front());
+        samples_.pop_front();
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, x.e0, x.e1) == hipSuccess && x.bytes) {
+            auto it = tuning_.find(x.sig);
+            if (it != tuning_.end() && !tuned_.count(x.sig)) {
+                Candidate &c = it->second.cands[x.cand];
+                const double v = ms * 1e6 / static_cast<double>(x.bytes);
+                if (c.best_ns_per_byte < 0 || v < c.best_ns_per_byte) c.best_ns_per_byte = v;
+                c.runs++;
+                const Candidate *best = nullptr;
+                bool all = true;
+                for (const Candidate &k : it->second.cands) {
+                    if (k.runs < kTuneRuns) all = false;
+                    if (k.best_ns_per_byte >= 0 && (!best || k.best_ns_per_byte < best->best_ns_per_byte)) best = &k;
+                }
+                if (all && best) {
+                    tuned_[x.sig] = {best->tile, best->cw};
+                    st_.tuned_signatures = static_cast<uint32_t>(tuned_.size());
+                    tuning_.erase(it);
+                }
+            }
+        }
+        event_pool_.push_back(x.e0);
+        event_pool_.push_back(x.e1);
+    }
 }
 
 scrs_batch_stats BatchEngine::stats() const {

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <deque>
 #include <list>
@@ -77,13 +78,42 @@ class BatchEngine {
         HostBuf host;        // staged header, ranges, solves, pointer and index arrays
         DevBuf dev;          // device copy of the staged bytes
         DevBuf scratch;      // solve work and fused coefficients
-        DevBuf counter;      // unit queue
+        DevBuf counter;      // unit queue (zero between launches)
         hipEvent_t done = nullptr;
         uint64_t seq = 0;
-        uint32_t generation = 0;  // bumps when `dev` / `scratch` move
-        // Graphs captured against this slot's buffers, per tile width.
+        uint32_t generation = 0;  // bumps when `dev` moves
+        // Multiply-XOR launches captured against this slot's buffers, per
+        // tile width.
         std::map<uint32_t, std::pair<uint32_t, hipGraphExec_t>> graphs;
     };
+    // Staging slots: a batch reuses the slot of the batch four back, so
+    // batches in flight on several streams rarely wait for a slot.
+    static constexpr uint32_t kSlots = 4;
+    // Online tuning of one batch signature: every candidate (tile bytes,
+    // codewords per unit) runs on real batches of that signature, timed with
+    // events, and the fastest per byte is kept once each has run.
+    struct Candidate {
+        uint32_t tile = 0, cw = 0;
+        double best_ns_per_byte = -1;
+        uint32_t runs = 0;
+    };
+    // Timed runs per candidate before the signature is decided (the best of
+    // them counts, so one run slowed by a neighbour does not decide).
+    static constexpr uint32_t kTuneRuns = 2;
+    struct Tuning {
+        std::vector<Candidate> cands;
+        uint32_t next = 0;
+    };
+    // Batch signature: log2 of the mean symbol length, GF(2^16) present,
+    // encode share in quarters, and whether an encode geometry repeats.
+    using Sig = std::array<uint32_t, 4>;
+    struct Sample {
+        Sig sig{};
+        uint32_t cand = 0;
+        uint64_t bytes = 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+    };
+    void harvest_samples();
     struct Plan;
 
     scrs_status validate(const scrs_job &j) const;
@@ -104,12 +134,14 @@ class BatchEngine {
     DevBuf coords_[3];   // Standard, Good, Tower coordinates of every wire index
     DevBuf gf8_;         // exp (512) + log (256)
     const uint16_t *gf16_log_exp_ = nullptr;  // GF(2^16) log / exp (shared per device)
-    Slot slots_[2];
+    Slot slots_[kSlots];
     uint32_t next_slot_ = 0;
     uint64_t seq_ = 0;
 
     DeviceArena arena_;
-    hipEvent_t last_done_ = nullptr;  // the previous batch, on whatever stream
+    // The previous batch's tables and recipes are written (its solve has
+    // run), on whatever stream: a batch's multiply-XOR may read them.
+    hipEvent_t last_solved_ = nullptr;
     // Resident caches, least recently used first.
     std::map<std::tuple<int32_t, uint64_t, uint64_t>, CacheEntry *> enc_cache_;
     std::unordered_map<std::vector<uint64_t>, CacheEntry *, RecipeKeyHash> recipe_cache_;
@@ -136,7 +168,10 @@ class BatchEngine {
     std::vector<std::tuple<int32_t, uint64_t, uint64_t, uint64_t>> sig_keys_;
 
     // Autotuner: batch signature -> (tile bytes, codewords per unit).
-    std::map<std::vector<uint32_t>, std::pair<uint32_t, uint32_t>> tuned_;
+    std::map<Sig, std::pair<uint32_t, uint32_t>> tuned_;
+    std::map<Sig, Tuning> tuning_;
+    std::deque<Sample> samples_;
+    std::vector<hipEvent_t> event_pool_;
     mutable std::mutex mu_;
     scrs_batch_stats st_{};
 };

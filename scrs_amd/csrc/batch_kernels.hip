@@ -143,9 +143,7 @@ __device__ void tables_job16(const BatchSolve &j, const FieldGf65536 &f) {
 }
 
 __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr, const uint8_t *gf8,
-                                                         const uint16_t *gf16_log_exp, uint32_t *counter) {
-    // The multiply-XOR launch that follows takes units from this counter.
-    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+                                                         const uint16_t *gf16_log_exp) {
     __shared__ uint8_t exp[512];
     __shared__ uint8_t log[256];
     for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) exp[i] = gf8[i];
@@ -196,6 +194,14 @@ __device__ __forceinline__ void unit8(const BatchRange &R, uint32_t cw0, uint32_
             for (uint32_t d = 0; d < D; d++) acc[b][d] = 0;
         for (uint32_t t0 = tb; t0 < te; t0 += kChunk8) {
             const uint32_t tc = min(kChunk8, te - t0);
+            // Two terms ahead: the term after next loads while this one
+            // folds; the first two are issued before the chunk's tables are
+            // staged, so their latency overlaps the staging's.
+            uint32_t wn[4], wm[4];
+            if (active) {
+                load16<VEC>(load_uniform(R.src + c * R.T + t0), p, R.L, wn);
+                if (tc > 1) load16<VEC>(load_uniform(R.src + c * R.T + t0 + 1), p, R.L, wm);
+            }
             if (!single || c == cw0) {
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < tc * 32 * D; i += TPU) {
@@ -207,14 +213,13 @@ __device__ __forceinline__ void unit8(const BatchRange &R, uint32_t cw0, uint32_
                 __syncthreads();
             }
             if (!active) continue;
-            // One term ahead: the next source loads while this one folds.
-            uint32_t wn[4];
-            load16<VEC>(load_uniform(R.src + c * R.T + t0), p, R.L, wn);
 #pragma unroll 1
             for (uint32_t t = 0; t < tc; t++) {
                 const uint32_t gt = c * R.T + t0 + t;
                 const uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
-                if (t + 1 < tc) load16<VEC>(load_uniform(R.src + gt + 1), p, R.L, wn);
+#pragma unroll
+                for (int i = 0; i < 4; i++) wn[i] = wm[i];
+                if (t + 2 < tc) load16<VEC>(load_uniform(R.src + gt + 2), p, R.L, wm);
                 if (DEC && do_copy) {
                     uint8_t *cp = load_uniform(R.copy + gt);
                     if (cp) store16<VEC>(cp, p, R.L, w);
@@ -256,6 +261,11 @@ __device__ __forceinline__ void unit16(const BatchRange &R, uint32_t cw0, uint32
             for (uint32_t d = 0; d < D; d++) acc[e][d] = 0;
         for (uint32_t t0 = tb; t0 < te; t0 += kChunk16) {
             const uint32_t tc = min(kChunk16, te - t0);
+            uint32_t wn[4], wm[4];  // two terms ahead, as in unit8
+            if (active) {
+                load16<VEC>(load_uniform(R.src + c * R.T + t0), p, R.L, wn);
+                if (tc > 1) load16<VEC>(load_uniform(R.src + c * R.T + t0 + 1), p, R.L, wm);
+            }
             if (!single || c == cw0) {
                 __syncthreads();
                 const uint8_t *tb = reinterpret_cast<const uint8_t *>(R.tables);
@@ -268,13 +278,13 @@ __device__ __forceinline__ void unit16(const BatchRange &R, uint32_t cw0, uint32
                 __syncthreads();
             }
             if (!active) continue;
-            uint32_t wn[4];
-            load16<VEC>(load_uniform(R.src + c * R.T + t0), p, R.L, wn);
 #pragma unroll 1
             for (uint32_t t = 0; t < tc; t++) {
                 const uint32_t gt = c * R.T + t0 + t;
                 const uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
-                if (t + 1 < tc) load16<VEC>(load_uniform(R.src + gt + 1), p, R.L, wn);
+#pragma unroll
+                for (int i = 0; i < 4; i++) wn[i] = wm[i];
+                if (t + 2 < tc) load16<VEC>(load_uniform(R.src + gt + 2), p, R.L, wm);
                 if (DEC && do_copy) {
                     uint8_t *cp = load_uniform(R.copy + gt);
                     if (cp) store16<VEC>(cp, p, R.L, w);
@@ -345,6 +355,10 @@ __device__ __forceinline__ BatchRange load_range(const BatchRange *p) {
     return r;
 }
 
+// counter[0] hands out units; counter[1] counts the workgroups that have
+// left the loop.  The last one to leave returns both to zero for the next
+// launch: every other workgroup has already drawn its final unit, so nothing
+// reads the counter after that.
 template <int TPU>
 __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr, uint32_t *counter) {
     __shared__ __align__(16) uint8_t lds[kLds];
@@ -355,7 +369,13 @@ __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr,
         if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t u = __builtin_amdgcn_readfirstlane(s_unit);
-        if (u >= nunits) break;
+        if (u >= nunits) {
+            if (threadIdx.x == 0 && atomicAdd(counter + 1, 1u) == gridDim.x - 1) {
+                __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(counter + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
         // Last range whose first unit is <= u.
         uint32_t lo = 0, hi = nranges - 1;
         while (lo < hi) {
@@ -384,8 +404,8 @@ __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr,
 }  // namespace
 
 hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, const uint16_t *gf16_log_exp,
-                              uint32_t *counter, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables, gf16_log_exp, counter);
+                              uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, hdr, gf8_tables, gf16_log_exp);
     return hipGetLastError();
 }
 

@@ -54,7 +54,13 @@ uint64_t footprint(const scrs_job &j) {
 struct scrs_server {
     scrs_server_config cfg{};
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // plain mode; batched mode's first stream
+    // Batched mode: consecutive batches go to the streams in turn, so one
+    // batch's copy and solve overlap the previous batch's multiply-XOR.
+    static constexpr int kBatchStreams = 2;
+    static constexpr size_t kBatchesInFlight = 3;  // below the engine's staging slots
+    hipStream_t streams[kBatchStreams] = {};
+    int next_stream = 0;
     scrs_batch_engine *eng = nullptr;
     // Plain mode: codecs per geometry and a ring of device pointer arrays.
     std::map<Key, scrs_batch_encoder *> encs;
@@ -160,8 +166,22 @@ void scrs_server::loop() {
     std::vector<scrs_job> jobs;
     while (true) {
         bool progressed = false;
-        // Completions, in launch order.
-        while (!inflight.empty() && hipEventQuery(inflight.front().ev) == hipSuccess) {
+        // Completions.  Plain mode runs on one stream, so its work finishes
+        // in launch order (the pointer ring relies on it); batches on
+        // several streams may finish in any order.
+        if (cfg.batched) {
+            for (auto it = inflight.begin(); it != inflight.end();) {
+                if (hipEventQuery(it->ev) != hipSuccess) {
+                    ++it;
+                    continue;
+                }
+                events.push_back(it->ev);
+                finish(*it, scrs_now_ns());
+                it = inflight.erase(it);
+                progressed = true;
+            }
+        }
+        while (!cfg.batched && !inflight.empty() && hipEventQuery(inflight.front().ev) == hipSuccess) {
             InFlight &f = inflight.front();
             ring_tail = f.ring_end ? f.ring_end : ring_tail;
             events.push_back(f.ev);
@@ -173,7 +193,7 @@ void scrs_server::loop() {
         // always admitted when nothing is in flight).
         std::vector<Request *> batch;
         uint64_t bytes = 0;
-        const size_t max_inflight = cfg.batched ? 2 : 1u << 20;
+        const size_t max_inflight = cfg.batched ? kBatchesInFlight : 1u << 20;
         if (inflight.size() < max_inflight) {
             std::lock_guard<std::mutex> lock(mu);
             while (!pending.empty() && batch.size() < (cfg.batched ? max_jobs : 1)) {
@@ -193,17 +213,22 @@ void scrs_server::loop() {
             f.reqs = batch;
             f.bytes = bytes;
             f.start = take_event();
-            (void)hipEventRecord(f.start, stream);
+            hipStream_t s = stream;
+            if (cfg.batched) {
+                s = streams[next_stream];
+                next_stream = (next_stream + 1) % kBatchStreams;
+            }
+            (void)hipEventRecord(f.start, s);
             if (cfg.batched) {
                 jobs.clear();
                 for (Request *r : batch) jobs.push_back(r->job);
                 uint64_t failed = 0;
-                f.status = scrs_batch_engine_submit(eng, jobs.data(), jobs.size(), stream, &failed);
+                f.status = scrs_batch_engine_submit(eng, jobs.data(), jobs.size(), s, &failed);
             } else {
                 f.status = plain_one(batch.front(), &f.ring_end);
             }
             f.ev = take_event();
-            (void)hipEventRecord(f.ev, stream);
+            (void)hipEventRecord(f.ev, s);
             const uint64_t t1 = scrs_now_ns();
             std::lock_guard<std::mutex> lock(mu);
             counters.batches++;
@@ -251,6 +276,9 @@ extern "C" scrs_status scrs_server_create(const scrs_server_config *cfg, scrs_se
     if (cfg) s->cfg = *cfg;
     (void)hipGetDevice(&s->device);
     hipError_t err = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    s->streams[0] = s->stream;
+    for (int i = 1; i < scrs_server::kBatchStreams && err == hipSuccess && s->cfg.batched; i++)
+        err = hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking);
     scrs_status r = ok();
     if (err == hipSuccess && s->cfg.batched) r = scrs_batch_engine_create(&s->cfg.batch, &s->eng);
     if (err == hipSuccess && !s->cfg.batched) {
@@ -259,6 +287,8 @@ extern "C" scrs_status scrs_server_create(const scrs_server_config *cfg, scrs_se
     }
     if (err != hipSuccess || r.code != SCRS_OK) {
         if (s->eng) scrs_batch_engine_destroy(s->eng);
+        for (int i = 1; i < scrs_server::kBatchStreams; i++)
+            if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
         return err != hipSuccess ? st(SCRS_ERR_DEVICE, err) : r;
@@ -283,12 +313,15 @@ extern "C" void scrs_server_destroy(scrs_server *s) {
         for (Request *r : f.reqs) delete r;
     }
     for (Request *r : s->pending) delete r;
-    (void)hipStreamSynchronize(s->stream);
+    for (int i = 0; i < scrs_server::kBatchStreams; i++)
+        if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     for (auto &kv : s->encs) scrs_batch_encoder_destroy(kv.second);
     for (auto &kv : s->incrs) scrs_incr_encoder_destroy(kv.second);
     for (auto &kv : s->decs) scrs_batch_decoder_destroy(kv.second);
     if (s->eng) scrs_batch_engine_destroy(s->eng);
+    for (int i = 1; i < scrs_server::kBatchStreams; i++)
+        if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
     (void)hipStreamDestroy(s->stream);
     delete s;
 }

@@ -153,19 +153,33 @@ def test_graph_replays_match_eager(torch_cuda):
 
 
 def test_autotuner_picks_a_configuration(torch_cuda):
+    """Online tuning: successive batches of one signature each run the next
+    candidate configuration (every output correct), and once every candidate
+    has been timed the fastest is kept for the later batches."""
     torch = torch_cuda
     w = Workload(torch, 7, 40, lens=(512, 4096))
     eng = BatchEngine(tile_bytes=0, cw_per_unit=0)
-    eng.submit(w.jobs)
-    torch.cuda.synchronize()
-    w.check()
-    st = eng.stats()
-    assert st["tuned_signatures"] == 1 and st["tile_bytes"] in (1024, 2048, 4096)
-    w.clear()
-    eng.submit(w.jobs)  # same signature: no tuning
-    torch.cuda.synchronize()
-    w.check()
-    assert eng.stats()["tuned_signatures"] == 1
+    seen = set()
+    for _ in range(30):  # at most 3 tiles x 4 codewords per unit, two runs each
+        w.clear()
+        eng.submit(w.jobs)
+        torch.cuda.synchronize()
+        w.check()
+        st = eng.stats()
+        seen.add((st["tile_bytes"], st["cw_per_unit"]))
+        if st["tuned_signatures"] == 1:
+            break
+    assert st["tuned_signatures"] == 1 and len(seen) > 1
+    chosen = []
+    for _ in range(3):  # the same signature: the tuned configuration, no more tuning
+        w.clear()
+        eng.submit(w.jobs)
+        torch.cuda.synchronize()
+        w.check()
+        st = eng.stats()
+        chosen.append((st["tile_bytes"], st["cw_per_unit"]))
+    assert st["tuned_signatures"] == 1 and len(set(chosen)) == 1
+    assert chosen[0][0] in (1024, 2048, 4096)
 
 
 def test_validation_rejects_before_enqueue(torch_cuda):

@@ -86,7 +86,7 @@ def test_batched_mode_equals_plain_mode(plain_outputs, torch_cuda):
     every output equals the same request run alone."""
     serving, torch = plain_outputs, torch_cuda
     serving.clear(0)
-    batched = Server(batched=True, max_batch_jobs=1024, use_graph=True, cache_bytes=16 << 30)
+    batched = Server(batched=True, max_batch_jobs=1024, use_graph=False, cache_bytes=16 << 30)
     try:
         _replay(batched, serving.jobs(0), rate=200_000)
         torch.cuda.synchronize()
