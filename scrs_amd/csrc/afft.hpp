@@ -24,6 +24,11 @@ enum AfftPassFlags : uint32_t {
     kAfftInv = 1,    // inverse levels over the pass bits (bottom-up)
     kAfftDeriv = 2,  // formal derivative contributions of the pass bits
     kAfftFwd = 4,    // forward levels over the pass bits (top-down)
+    // The derivative with every factor 1: plain XORs of the rows.  With the
+    // coefficients scaled by T(q) = prod over bits b of q of d_b before and
+    // the sums by 1/T(p) after, it equals the derivative with the factors d_b:
+    // sum over b clear in p of d_b C[p | 2^b] = (1/T(p)) sum of T(p | 2^b) C[p | 2^b].
+    kAfftDerivUnit = 8,
 };
 
 struct AfftPassArgs {
@@ -46,6 +51,9 @@ struct AfftPassArgs {
     // the load / before the store.
     const uint16_t *scale_in = nullptr;
     const uint16_t *scale_out = nullptr;
+    // Per-row tables applied to the derivative (after the accumulator,
+    // before the forward levels).
+    const uint16_t *scale_der = nullptr;
     const uint16_t *node_tabs = nullptr;   // 65536 node tables
     const uint16_t *deriv_tabs = nullptr;  // 16 derivative tables
     uint64_t L = 0;
@@ -60,7 +68,7 @@ struct AfftPassArgs {
 // Launch one pass over all 2^D rows (groups = 2^(D-g)) for `count`
 // codewords.
 hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, hipStream_t s);
-size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out);
+size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der);
 
 // Nibble product tables: out[i*64 + q*16 + n] = c[i] * (n << 4q).
 hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s);
@@ -99,6 +107,9 @@ hipError_t launch_afft_scatter_ptrs(const uint16_t *indices, const uint8_t *cons
 // Per-device constant tables.
 struct AfftDeviceTables {
     DevBuf node_tabs, deriv_tabs, exp;
+    // Row tables of T(q) = prod over the bits b of q of the derivative
+    // factor d_b, and of 1/T(q), for q < 65536 (kAfftDerivUnit).
+    DevBuf deriv_scale, deriv_unscale;
 };
 hipError_t afft_device_tables(const AfftDeviceTables **out);
 

@@ -12,6 +12,14 @@
 // the forward transform, and P(e) = (Lambda P)'(e) / Lambda'(e).  The encode
 // of other shapes is the same computation with the data points known and
 // the repair points as targets.
+//
+// Above one pass the derivative runs with unit factors: the last inverse
+// pass scales coefficient row q by T(q), the product of the derivative
+// factors d_b over the bits b of q, the derivative passes only XOR rows, and
+// the pass that completes it scales row p by 1/T(p) before the forward
+// levels.  Since T(p | 2^b) = T(p) d_b for a bit b clear in p, that equals
+// the reference's sum of d_b C[p | 2^b] (transform.rs:188-213) with two
+// products per row instead of one per term.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -81,6 +89,25 @@ hipError_t afft_device_tables(const AfftDeviceTables **out) {
     auto *t = new AfftDeviceTables();
     DevBuf consts;
     std::vector<uint16_t> d(m.deriv, m.deriv + kAfftMaxLog);
+    // T(q) and 1/T(q) through logarithms; every derivative factor is
+    // nonzero (the linear coefficient of a normalised subspace polynomial).
+    std::vector<uint16_t> dscale(65536), dunscale(65536);
+    for (uint32_t q = 0; q < 65536; q++) {
+        uint32_t lg = 0;
+        for (uint32_t b = 0; b < kAfftMaxLog; b++)
+            if ((q >> b) & 1u) lg = (lg + m.log[m.deriv[b]]) % kAfftOrder;
+        dscale[q] = m.exp[lg];
+        dunscale[q] = m.exp[(kAfftOrder - lg) % kAfftOrder];
+    }
+    auto build = [&](const std::vector<uint16_t> &c, DevBuf &tabs) -> hipError_t {
+        hipError_t e;
+        if ((e = tabs.ensure(c.size() * 128)) != hipSuccess ||
+            (e = hipMemcpy(consts.ptr, c.data(), c.size() * 2, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = launch_afft_tables(consts.as<uint16_t>(), static_cast<uint32_t>(c.size()), tabs.as<uint16_t>(),
+                                    nullptr)) != hipSuccess)
+            return e;
+        return hipStreamSynchronize(nullptr);
+    };
     if ((err = consts.ensure(65536 * 2)) == hipSuccess &&
         (err = t->node_tabs.ensure(65536ull * 128)) == hipSuccess &&
         (err = t->deriv_tabs.ensure(kAfftMaxLog * 128)) == hipSuccess &&
@@ -92,8 +119,9 @@ hipError_t afft_device_tables(const AfftDeviceTables **out) {
         (err = hipStreamSynchronize(nullptr)) == hipSuccess &&
         (err = hipMemcpy(consts.ptr, d.data(), kAfftMaxLog * 2, hipMemcpyHostToDevice)) == hipSuccess &&
         (err = launch_afft_tables(consts.as<uint16_t>(), kAfftMaxLog, t->deriv_tabs.as<uint16_t>(), nullptr)) ==
-            hipSuccess)
-        err = hipStreamSynchronize(nullptr);
+            hipSuccess &&
+        (err = hipStreamSynchronize(nullptr)) == hipSuccess && (err = build(dscale, t->deriv_scale)) == hipSuccess)
+        err = build(dunscale, t->deriv_unscale);
     if (err != hipSuccess) {
         delete t;
         return err;
@@ -274,6 +302,7 @@ scrs_status AfftEngine::encode(const uint8_t *data, uint8_t *repairs, uint64_t c
             } else {
                 set_in_work(a, w1);
             }
+            if (i + 1 == ns) a.scale_out = dev->deriv_scale.as<uint16_t>();  // unit-factor derivative
             set_out_work(a, w1);
             if ((err = launch_afft_pass(a, D, cn, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
         }
@@ -281,7 +310,7 @@ scrs_status AfftEngine::encode(const uint8_t *data, uint8_t *repairs, uint64_t c
             AfftPassArgs a = base;
             a.b_lo = parts[i].first;
             a.g = parts[i].second;
-            a.flags = kAfftDeriv;
+            a.flags = kAfftDerivUnit;
             set_in_work(a, w1);
             if (i > 0) {
                 a.aux_base = w2;
@@ -296,7 +325,8 @@ scrs_status AfftEngine::encode(const uint8_t *data, uint8_t *repairs, uint64_t c
             a.g = parts[i].second;
             a.row_limit = n;
             if (i + 1 == static_cast<int>(ns)) {
-                a.flags = kAfftDeriv | kAfftFwd;
+                a.flags = kAfftDerivUnit | kAfftFwd;
+                a.scale_der = dev->deriv_unscale.as<uint16_t>();
                 set_in_work(a, w1);
                 if (ns > 1) {  // lower parts' derivative contributions
                     a.aux_base = w2;
@@ -432,6 +462,7 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
         } else {
             in_work(a, w1);
         }
+        if (i + 1 == ns) a.scale_out = dev->deriv_scale.as<uint16_t>();  // unit-factor derivative
         out_work(a, w1);
         if ((err = launch_afft_pass(a, D, 1, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     }
@@ -439,7 +470,7 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
         AfftPassArgs a = base;
         a.b_lo = parts[i].first;
         a.g = parts[i].second;
-        a.flags = kAfftDeriv;
+        a.flags = kAfftDerivUnit;
         in_work(a, w1);
         if (i > 0) a.aux_base = w2;
         out_work(a, w2);
@@ -451,7 +482,8 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
         a.g = parts[i].second;
         a.row_limit = k;
         if (i + 1 == static_cast<int>(ns)) {
-            a.flags = kAfftDeriv | kAfftFwd;
+            a.flags = kAfftDerivUnit | kAfftFwd;
+            a.scale_der = dev->deriv_unscale.as<uint16_t>();
             in_work(a, w1);
             if (ns > 1) a.aux_base = w2;  // lower parts' derivative contributions
         } else {

@@ -105,21 +105,26 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 // One workgroup owns a row group and `tpb` consecutive column tiles: the
 // node tables are staged once and the tiles run in turn, the next tile's
 // rows loading while the current one computes.
-template <int R, bool A4>
+// GL: the pass's level count fixed at compile time (0: taken from the
+// arguments), so a full pass's stage layouts and table offsets fold into
+// constants.
+template <int R, bool A4, uint32_t GL>
 __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_t tpb) {
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
     extern __shared__ __align__(128) uint32_t smem[];  // tables at multiples of 128 bytes
-    const uint32_t g = a.g, G = 1u << g, b_lo = a.b_lo;
+    const uint32_t g = GL ? GL : a.g, G = 1u << g, b_lo = a.b_lo;
     const uint32_t gidx = blockIdx.y;
     const uint32_t base = (gidx & ((1u << b_lo) - 1u)) | ((gidx >> b_lo) << (b_lo + g));
     if (a.row_limit && (base >> b_lo) > ((a.row_limit - 1u) >> b_lo)) return;
-    const bool inv = a.flags & kAfftInv, der = a.flags & kAfftDeriv, fwd = a.flags & kAfftFwd;
+    const bool inv = a.flags & kAfftInv, fwd = a.flags & kAfftFwd;
+    const bool unit = a.flags & kAfftDerivUnit, der = unit || (a.flags & kAfftDeriv);
     uint32_t *data = smem;
     uint16_t *inv_t = reinterpret_cast<uint16_t *>(smem + G * kTileDwords);
     uint16_t *fwd_t = inv_t + (inv ? (G - 1) * kTab : 0);
     uint16_t *der_t = fwd_t + (fwd ? (G - 1) * kTab : 0);
-    uint16_t *sin_t = der_t + (der ? 16 * kTab : 0);
+    uint16_t *sin_t = der_t + (der && !unit ? 16 * kTab : 0);
     uint16_t *sout_t = sin_t + (a.scale_in ? G * kTab : 0);
+    uint16_t *sder_t = sout_t + (a.scale_out ? G * kTab : 0);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
 
     // Stage the node tables of this group: level beta (pass bit b_lo+beta)
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     };
     if (inv) stage_nodes(inv_t, a.inv_base + base);
     if (fwd) stage_nodes(fwd_t, a.fwd_base + base);
-    if (der) {
+    if (der && !unit) {
         const uint32_t *s32 = reinterpret_cast<const uint32_t *>(a.deriv_tabs);
         uint32_t *d32 = reinterpret_cast<uint32_t *>(der_t);
         for (uint32_t i = tid; i < 16 * (kTab / 2); i += nthr) d32[i] = s32[i];
@@ -152,6 +157,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     };
     if (a.scale_in) stage_rows(sin_t, a.scale_in);
     if (a.scale_out) stage_rows(sout_t, a.scale_out);
+    if (a.scale_der) stage_rows(sder_t, a.scale_der);
     __syncthreads();
 
     const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
@@ -256,10 +262,28 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     const uint32_t tiles = static_cast<uint32_t>((L + kTileDwords * 4 - 1) / (kTileDwords * 4));
     const uint32_t t_first = blockIdx.x * tpb, t_end = min(tiles, t_first + tpb);
     const uint32_t k_in = inv ? 0u : nst - 1;  // stage layout the rows load in
+    // The derivative starts in the last stage's layout and, with two stages,
+    // ends in the first one's: either way the accumulator rows are added in
+    // stage layout 0.
+    const uint32_t k_der = 0;
     if (t_first < t_end) load_raw(stage_lo(k_in), stage_t(k_in), t_first, nx);
+    uint32_t ax[R];
     for (uint32_t tile = t_first; tile < t_end; tile++) {
         cb = (static_cast<uint64_t>(tile) * kTileDwords + lane) * 4u;
         valid = cb < L;
+        // The derivative accumulator of this tile loads now; its latency
+        // hides behind the stages that run before it is added.
+        if (der && a.aux_base) {
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                const uint32_t p = rho<R>(wv, stage_lo(k_der), stage_t(k_der), j);
+                const uint8_t *row = a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(base | (p << b_lo)) * L;
+                if constexpr (A4)
+                    ax[j] = ld_col4(row, a.node_tabs, static_cast<uint32_t>(cb), L);
+                else
+                    ax[j] = valid ? ld_col(row, cb, L) : 0u;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = nx[j];
         if (tile + 1 < t_end) load_raw(stage_lo(k_in), stage_t(k_in), tile + 1, nx);
@@ -287,10 +311,19 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
             // Source-major: each coefficient c[i] is scaled by the table of
             // every stage bit set in i and added to the row with that bit
             // clear, so its nibble offsets serve all of its products (the
-            // tables of one stage are consecutive).
+            // tables of one stage are consecutive).  With unit factors the
+            // terms are plain XORs.
             auto der_stage = [&](const uint32_t(&c)[R], uint32_t(&d)[R], uint32_t s_lo, uint32_t t) {
 #pragma unroll
                 for (int j = 0; j < R; j++) d[j] = 0;
+                if (unit) {
+#pragma unroll
+                    for (int i = 1; i < R; i++)
+#pragma unroll
+                        for (int bs = 0; bs < static_cast<int>(RB); bs++)
+                            if (((i >> bs) & 1) && static_cast<uint32_t>(bs) < t) d[i ^ (1 << bs)] ^= c[i];
+                    return;
+                }
                 const uint16_t *tab0 = der_t + (b_lo + s_lo) * kTab;
 #pragma unroll
                 for (int i = 1; i < R; i++) {
@@ -326,18 +359,11 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
                 __syncthreads();  // the tile is free for the next layout change
                 cur = static_cast<int>(k);
             }
+            // Here cur == k_der: the accumulator rows loaded in this layout.
 #pragma unroll
             for (int j = 0; j < R; j++) {
-                if (a.aux_base && (A4 || valid)) {
-                    const uint32_t p = rho<R>(wv, stage_lo(cur), stage_t(cur), j);
-                    const uint32_t r = base | (p << b_lo);
-                    const uint8_t *row = a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(r) * L;
-                    if constexpr (A4)
-                        acc[j] ^= ld_col4(row, a.node_tabs, static_cast<uint32_t>(cb), L);
-                    else
-                        acc[j] ^= ld_col(row, cb, L);
-                }
-                v[j] = acc[j];
+                const uint32_t x = a.aux_base ? acc[j] ^ ax[j] : acc[j];
+                v[j] = a.scale_der ? gmul2(sder_t + rho<R>(wv, stage_lo(cur), stage_t(cur), j) * kTab, x) : x;
             }
         }
         if (fwd) {
@@ -355,7 +381,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     }
 }
 
-template <int R, bool A4>
+template <int R, bool A4, uint32_t GL = 0>
 hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
     const uint32_t G = 1u << a.g;
     const uint64_t tiles = (a.L + kTileDwords * 4 - 1) / (kTileDwords * 4);
@@ -368,9 +394,9 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4>), 160 * 1024);
+    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL>), 160 * 1024);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((afft_pass_kernel<R, A4>), grid, block, lds, s, a, tpb);
+    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL>), grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
 
@@ -488,21 +514,24 @@ __global__ __launch_bounds__(256) void afft_scatter_kernel(const uint16_t *idx, 
 
 }  // namespace
 
-size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out) {
+size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der) {
     const size_t G = size_t{1} << g, tab = kTab * 2;
     size_t b = G * kTileDwords * 4;
     if (flags & kAfftInv) b += (G - 1) * tab;
     if (flags & kAfftFwd) b += (G - 1) * tab;
-    if (flags & kAfftDeriv) b += 16 * tab;
+    if ((flags & kAfftDeriv) && !(flags & kAfftDerivUnit)) b += 16 * tab;
     if (scale_in) b += G * tab;
     if (scale_out) b += G * tab;
+    if (scale_der) b += G * tab;
     return b;
 }
 
 hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, hipStream_t s) {
     if (a.g == 0 || a.g > 8 || a.b_lo + a.g > D || D > kAfftMaxLog || count == 0 || count > 65535 || a.L == 0)
         return hipErrorInvalidValue;
-    const size_t lds = afft_pass_lds_bytes(a.g, a.flags, a.scale_in != nullptr, a.scale_out != nullptr);
+    if (a.scale_der && !(a.flags & (kAfftDeriv | kAfftDerivUnit))) return hipErrorInvalidValue;
+    const size_t lds =
+        afft_pass_lds_bytes(a.g, a.flags, a.scale_in != nullptr, a.scale_out != nullptr, a.scale_der != nullptr);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     // Dword buffer instructions when every row the pass touches starts on a
     // multiple of 4 (row tables: as declared by the caller).
@@ -512,6 +541,7 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
                     (a.in_cw_stride | a.out_cw_stride | a.aux_cw_stride) % 4 == 0 && (!tables || a.rows_aligned4);
     if (a4) {
         switch (a.g) {
+            case 8: return launch_pass_r<16, true, 8>(a, D, count, lds, s);
             case 1: return launch_pass_r<2, true>(a, D, count, lds, s);
             case 2: return launch_pass_r<4, true>(a, D, count, lds, s);
             case 3: return launch_pass_r<8, true>(a, D, count, lds, s);
