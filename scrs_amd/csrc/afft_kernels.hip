@@ -27,6 +27,7 @@ namespace {
 
 constexpr uint32_t kTileDwords = 64;  // 256-byte column tile
 constexpr uint32_t kTab = 64;         // u16 entries per constant table
+constexpr uint32_t kTabW = 128;       // u16 entries per wide constant table
 
 typedef __attribute__((address_space(4))) const uint64_t *const_u64_ptr;
 template <typename T>
@@ -55,6 +56,28 @@ __device__ __forceinline__ uint32_t gmul(const uint16_t *t, uint32_t x) {
 // Two packed elements times the same constant.
 __device__ __forceinline__ uint32_t gmul2(const uint16_t *t, uint32_t w) {
     return gmul(t, w & 0xffffu) | (gmul(t, w >> 16) << 16);
+}
+
+// Wide tables split an element 6 + 5 + 5 bits: entries [0, 64) are c*n,
+// [64, 96) c*(n << 6), [96, 128) c*(n << 11).  A 64-entry part spans the 32
+// banks once and a 32-entry part 16 of them, so lookups stay conflict-free
+// with three reads per element instead of four.  Tables start on multiples
+// of 256 bytes.
+__device__ __forceinline__ uint32_t gmulw(const uint16_t *t, uint32_t x) {
+    const uint32_t a = lds_byte(t);
+    __builtin_assume((a & 255u) == 0);
+    return lds_u16(a | ((x << 1) & 0x7Eu)) ^ lds_u16((a + 128u) | ((x >> 5) & 0x3Eu)) ^
+           lds_u16((a + 192u) | ((x >> 10) & 0x3Eu));
+}
+__device__ __forceinline__ uint32_t gmul2w(const uint16_t *t, uint32_t w) {
+    return gmulw(t, w & 0xffffu) | (gmulw(t, w >> 16) << 16);
+}
+template <bool W>
+__device__ __forceinline__ uint32_t gmul2n(const uint16_t *t, uint32_t w) {
+    if constexpr (W)
+        return gmul2w(t, w);
+    else
+        return gmul2(t, w);
 }
 
 __device__ __forceinline__ uint32_t ld_col(const uint8_t *p, uint64_t cb, uint64_t L) {
@@ -107,11 +130,12 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 // rows loading while the current one computes.
 // GL: the pass's level count fixed at compile time (0: taken from the
 // arguments), so a full pass's stage layouts and table offsets fold into
-// constants.
-template <int R, bool A4, uint32_t GL>
+// constants.  W: node tables in the wide format.
+template <int R, bool A4, uint32_t GL, bool W>
 __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_t tpb) {
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
-    extern __shared__ __align__(128) uint32_t smem[];  // tables at multiples of 128 bytes
+    constexpr uint32_t kTN = W ? kTabW : kTab;          // u16 entries per node table
+    extern __shared__ __align__(256) uint32_t smem[];  // tables at multiples of 128 (wide: 256) bytes
     const uint32_t g = GL ? GL : a.g, G = 1u << g, b_lo = a.b_lo;
     const uint32_t gidx = blockIdx.y;
     const uint32_t base = (gidx & ((1u << b_lo) - 1u)) | ((gidx >> b_lo) << (b_lo + g));
@@ -120,8 +144,8 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     const bool unit = a.flags & kAfftDerivUnit, der = unit || (a.flags & kAfftDeriv);
     uint32_t *data = smem;
     uint16_t *inv_t = reinterpret_cast<uint16_t *>(smem + G * kTileDwords);
-    uint16_t *fwd_t = inv_t + (inv ? (G - 1) * kTab : 0);
-    uint16_t *der_t = fwd_t + (fwd ? (G - 1) * kTab : 0);
+    uint16_t *fwd_t = inv_t + (inv ? (G - 1) * kTN : 0);
+    uint16_t *der_t = fwd_t + (fwd ? (G - 1) * kTN : 0);
     uint16_t *sin_t = der_t + (der && !unit ? 16 * kTab : 0);
     uint16_t *sout_t = sin_t + (a.scale_in ? G * kTab : 0);
     uint16_t *sder_t = sout_t + (a.scale_out ? G * kTab : 0);
@@ -131,13 +155,13 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     // uses G >> (beta+1) consecutive nodes.
     auto stage_nodes = [&](uint16_t *dst, uint32_t pbase) {
         uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(a.node_tabs);
+        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(W ? a.node_tabs_w : a.node_tabs);
         for (uint32_t beta = 0; beta < g; beta++) {
             const uint32_t B = b_lo + beta;
             const uint32_t cnt = G >> (beta + 1);
             const uint32_t first = (1u << (15u - B)) + (pbase >> (B + 1));
-            const uint32_t off = (G - (G >> beta)) * (kTab / 2);
-            for (uint32_t i = tid; i < cnt * (kTab / 2); i += nthr) d32[off + i] = s32[first * (kTab / 2) + i];
+            const uint32_t off = (G - (G >> beta)) * (kTN / 2);
+            for (uint32_t i = tid; i < cnt * (kTN / 2); i += nthr) d32[off + i] = s32[first * (kTN / 2) + i];
         }
     };
     if (inv) stage_nodes(inv_t, a.inv_base + base);
@@ -228,7 +252,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
         for (int j = 0; j < R; j++) data[rho<R>(wv, s_lo, t, j) * kTileDwords + lane] = v[j];
     };
     auto node_tab = [&](const uint16_t *tabs, uint32_t beta, uint32_t low_row) {
-        return tabs + ((G - (G >> beta)) + (low_row >> (beta + 1))) * kTab;
+        return tabs + ((G - (G >> beta)) + (low_row >> (beta + 1))) * kTN;
     };
     auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
 #pragma unroll
@@ -239,7 +263,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
                 if (j & (1 << bs)) continue;
                 const uint16_t *tab = node_tab(inv_t, s_lo + bs, rho<R>(wv, s_lo, t, j));
                 v[j + (1 << bs)] ^= v[j];
-                v[j] ^= gmul2(tab, v[j + (1 << bs)]);
+                v[j] ^= gmul2n<W>(tab, v[j + (1 << bs)]);
             }
         }
     };
@@ -251,7 +275,7 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
             for (int j = 0; j < R; j++) {
                 if (j & (1 << bs)) continue;
                 const uint16_t *tab = node_tab(fwd_t, s_lo + bs, rho<R>(wv, s_lo, t, j));
-                v[j] ^= gmul2(tab, v[j + (1 << bs)]);
+                v[j] ^= gmul2n<W>(tab, v[j + (1 << bs)]);
                 v[j + (1 << bs)] ^= v[j];
             }
         }
@@ -381,11 +405,16 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     }
 }
 
-template <int R, bool A4, uint32_t GL = 0>
+template <int R, bool A4, uint32_t GL = 0, bool W = false>
 hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
     const uint32_t G = 1u << a.g;
     const uint64_t tiles = (a.L + kTileDwords * 4 - 1) / (kTileDwords * 4);
-    const uint64_t groups = uint64_t{1} << (D - a.g);
+    // Groups whose rows all lie at or above row_limit are not launched:
+    // group gidx starts at row ((gidx >> b_lo) << (b_lo + g)) | (gidx & (2^b_lo - 1)),
+    // so the live groups are the first ((row_limit - 1) >> (b_lo + g)) + 1 blocks of 2^b_lo.
+    uint64_t groups = uint64_t{1} << (D - a.g);
+    if (a.row_limit)
+        groups = std::min<uint64_t>(groups, ((uint64_t{(a.row_limit - 1u) >> (a.b_lo + a.g)}) + 1) << a.b_lo);
     // Tiles per workgroup: enough workgroups to fill every CU to its LDS
     // capacity, no more, so each group's tables are staged as few times as
     // possible.
@@ -394,9 +423,9 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL>), 160 * 1024);
+    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W>), 160 * 1024);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL>), grid, block, lds, s, a, tpb);
+    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL, W>), grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
 
@@ -434,6 +463,32 @@ __global__ __launch_bounds__(256) void afft_tables_kernel(const uint16_t *c, uin
             }
             o[(q * 16 + n) / 2] = e0 | (e1 << 16);
         }
+}
+
+// Wide tables (the 6 + 5 + 5 split of gmulw).
+__global__ __launch_bounds__(256) void afft_tables_wide_kernel(const uint16_t *c, uint32_t count, uint16_t *out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    uint32_t basis[16];
+    basis[0] = c[i];
+    for (int b = 1; b < 8; b++) basis[b] = xtime16(basis[b - 1]);
+    basis[8] = times_u(c[i]);
+    for (int b = 9; b < 16; b++) basis[b] = xtime16(basis[b - 1]);
+    uint32_t *o = reinterpret_cast<uint32_t *>(out + static_cast<uint64_t>(i) * kTabW);
+    // Entries [first, first + 2^bits): the products with n << bit0.
+    auto part = [&](uint32_t first, int bit0, int bits) {
+        for (int n = 0; n < (1 << bits); n += 2) {
+            uint32_t e0 = 0, e1 = 0;
+            for (int b = 0; b < bits; b++) {
+                if ((n >> b) & 1) e0 ^= basis[bit0 + b];
+                if (((n + 1) >> b) & 1) e1 ^= basis[bit0 + b];
+            }
+            o[(first + n) / 2] = e0 | (e1 << 16);
+        }
+    };
+    part(0, 0, 6);
+    part(64, 6, 5);
+    part(96, 11, 5);
 }
 
 // ---- locator (afft/decoder.rs:672-771) -------------------------------------
@@ -514,11 +569,11 @@ __global__ __launch_bounds__(256) void afft_scatter_kernel(const uint16_t *idx, 
 
 }  // namespace
 
-size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der) {
-    const size_t G = size_t{1} << g, tab = kTab * 2;
+size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der, bool wide) {
+    const size_t G = size_t{1} << g, tab = kTab * 2, ntab = (wide ? kTabW : kTab) * 2;
     size_t b = G * kTileDwords * 4;
-    if (flags & kAfftInv) b += (G - 1) * tab;
-    if (flags & kAfftFwd) b += (G - 1) * tab;
+    if (flags & kAfftInv) b += (G - 1) * ntab;
+    if (flags & kAfftFwd) b += (G - 1) * ntab;
     if ((flags & kAfftDeriv) && !(flags & kAfftDerivUnit)) b += 16 * tab;
     if (scale_in) b += G * tab;
     if (scale_out) b += G * tab;
@@ -530,15 +585,21 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     if (a.g == 0 || a.g > 8 || a.b_lo + a.g > D || D > kAfftMaxLog || count == 0 || count > 65535 || a.L == 0)
         return hipErrorInvalidValue;
     if (a.scale_der && !(a.flags & (kAfftDeriv | kAfftDerivUnit))) return hipErrorInvalidValue;
-    const size_t lds =
-        afft_pass_lds_bytes(a.g, a.flags, a.scale_in != nullptr, a.scale_out != nullptr, a.scale_der != nullptr);
+    const bool sin = a.scale_in != nullptr, sout = a.scale_out != nullptr, sder = a.scale_der != nullptr;
+    const size_t lds = afft_pass_lds_bytes(a.g, a.flags, sin, sout, sder, false);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const size_t lds_w = afft_pass_lds_bytes(a.g, a.flags, sin, sout, sder, true);
     // Dword buffer instructions when every row the pass touches starts on a
     // multiple of 4 (row tables: as declared by the caller).
     auto al4 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; };
     const bool tables = a.in_rows || a.copy_rows || a.out_rows;
     const bool a4 = a.L % 4 == 0 && a.L <= 0x7fffffffull && al4(a.in_base) && al4(a.out_base) && al4(a.aux_base) &&
                     (a.in_cw_stride | a.out_cw_stride | a.aux_cw_stride) % 4 == 0 && (!tables || a.rows_aligned4);
+    // Wide node tables for the 16-row passes whose tables still fit.
+    if (a4 && a.g >= 4 && a.node_tabs_w && lds_w <= 160 * 1024) {
+        if (a.g == 8) return launch_pass_r<16, true, 8, true>(a, D, count, lds_w, s);
+        return launch_pass_r<16, true, 0, true>(a, D, count, lds_w, s);
+    }
     if (a4) {
         switch (a.g) {
             case 8: return launch_pass_r<16, true, 8>(a, D, count, lds, s);
@@ -559,6 +620,12 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
 hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(afft_tables_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_afft_tables_wide(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(afft_tables_wide_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c, count, out);
     return hipGetLastError();
 }
 
