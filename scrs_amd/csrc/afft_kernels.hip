@@ -80,6 +80,92 @@ __device__ __forceinline__ uint32_t gmul2n(const uint16_t *t, uint32_t w) {
         return gmul2(t, w);
 }
 
+// The same products for the table at byte address `base + off`, where `base`
+// is wave-uniform and aligned like a table and `off` is a compile-time
+// multiple of the table size: the lookup address is (base | entry) + off,
+// so `off` and the part offsets ride in the instruction's offset field and
+// every table of a level shares one scalar base.
+// The caller hides `base`'s value from the optimiser (an empty asm operand):
+// knowing its alignment, the compiler would turn each OR into an add and
+// move `off` and the part offsets back into scalar adds per table part.
+template <bool W>
+__device__ __forceinline__ uint32_t gmul1_at(uint32_t base, uint32_t off, uint32_t x) {
+    if constexpr (W) {
+        return lds_u16((base | ((x << 1) & 0x7Eu)) + off) ^ lds_u16((base | ((x >> 5) & 0x3Eu)) + off + 128u) ^
+               lds_u16((base | ((x >> 10) & 0x3Eu)) + off + 192u);
+    } else {
+        return lds_u16((base | ((x << 1) & 0x1Eu)) + off) ^ lds_u16((base | ((x >> 3) & 0x1Eu)) + off + 32u) ^
+               lds_u16((base | ((x >> 7) & 0x1Eu)) + off + 64u) ^ lds_u16((base | ((x >> 11) & 0x1Eu)) + off + 96u);
+    }
+}
+template <bool W>
+__device__ __forceinline__ uint32_t gmul2_at(uint32_t base, uint32_t off, uint32_t w) {
+    return gmul1_at<W>(base, off, w & 0xffffu) | (gmul1_at<W>(base, off, w >> 16) << 16);
+}
+
+// gmul2_at split in two halves, so that a level's products can be software
+// pipelined: the table reads of one product are issued before the previous
+// product's reads are combined.
+#ifndef SCRS_AFFT_PIPE
+#define SCRS_AFFT_PIPE 1  // products in flight ahead of the one being combined
+#endif
+constexpr int kPipe = SCRS_AFFT_PIPE;
+
+template <bool W>
+struct Parts {
+    static constexpr int N = W ? 6 : 8;
+    uint32_t r[N];
+};
+// Entry address of the field of `bits` bits at bit `lo` of w: one bit-field
+// extract and one shift-OR into the table base.  Written out because the
+// compiler rewrites the pair as shift + AND + OR, one instruction more
+// where the mask is not an inline constant.  Two ALU instructions, no
+// memory access.
+template <uint32_t lo, uint32_t bits>
+__device__ __forceinline__ uint32_t entry(uint32_t base, uint32_t w) {
+    uint32_t a;
+    asm("v_bfe_u32 %0, %1, %2, %3\n\tv_lshl_or_b32 %0, %0, 1, %4" : "=&v"(a) : "v"(w), "i"(lo), "i"(bits), "s"(base));
+    return a;
+}
+template <bool W>
+__device__ __forceinline__ void gmul2_issue(Parts<W> &p, uint32_t base, uint32_t off, uint32_t w) {
+    const uint32_t x0 = w & 0xffffu, x1 = w >> 16;
+    if constexpr (W) {
+        p.r[0] = lds_u16(entry<0, 6>(base, w) + off);
+        p.r[1] = lds_u16(entry<6, 5>(base, w) + off + 128u);
+        p.r[2] = lds_u16(entry<11, 5>(base, w) + off + 192u);
+        p.r[3] = lds_u16(entry<16, 6>(base, w) + off);
+        p.r[4] = lds_u16(entry<22, 5>(base, w) + off + 128u);
+        p.r[5] = lds_u16(entry<27, 5>(base, w) + off + 192u);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t x = e ? x1 : x0;
+            p.r[4 * e + 0] = lds_u16((base | ((x << 1) & 0x1Eu)) + off);
+            p.r[4 * e + 1] = lds_u16((base | ((x >> 3) & 0x1Eu)) + off + 32u);
+            p.r[4 * e + 2] = lds_u16((base | ((x >> 7) & 0x1Eu)) + off + 64u);
+            p.r[4 * e + 3] = lds_u16((base | ((x >> 11) & 0x1Eu)) + off + 96u);
+        }
+    }
+}
+template <bool W>
+__device__ __forceinline__ uint32_t gmul2_combine(const Parts<W> &p) {
+    // The reads zero-extend, so the two 16-bit sums pack with one shift-OR:
+    // two three-way XORs and the shift-OR (the compiler's 16-bit form takes
+    // four instructions).
+    if constexpr (W) {
+        uint32_t lo, hi;
+        asm("v_bitop3_b32 %0, %2, %3, %4 bitop3:0x96\n\t"
+            "v_bitop3_b32 %1, %5, %6, %7 bitop3:0x96\n\t"
+            "v_lshl_or_b32 %0, %1, 16, %0"
+            : "=&v"(lo), "=&v"(hi)
+            : "v"(p.r[0]), "v"(p.r[1]), "v"(p.r[2]), "v"(p.r[3]), "v"(p.r[4]), "v"(p.r[5]));
+        return lo;
+    }
+    else
+        return (p.r[0] ^ p.r[1] ^ p.r[2] ^ p.r[3]) | ((p.r[4] ^ p.r[5] ^ p.r[6] ^ p.r[7]) << 16);
+}
+
 __device__ __forceinline__ uint32_t ld_col(const uint8_t *p, uint64_t cb, uint64_t L) {
     if (((reinterpret_cast<uintptr_t>(p) | L) & 3u) == 0) return *reinterpret_cast<const uint32_t *>(p + cb);
     uint32_t w = 0;
@@ -131,8 +217,15 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 // GL: the pass's level count fixed at compile time (0: taken from the
 // arguments), so a full pass's stage layouts and table offsets fold into
 // constants.  W: node tables in the wide format.
+#ifndef SCRS_AFFT_WAVES_PER_EU
+#define SCRS_AFFT_WAVES_PER_EU 4
+#endif
+// A 16-row pass stages at least 64 KiB of LDS, so a CU holds one 1024-thread
+// workgroup: 4 waves per SIMD.  Telling the compiler so lets it spend the
+// registers of that occupancy on table reads in flight.
 template <int R, bool A4, uint32_t GL, bool W>
-__global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_t tpb) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_WAVES_PER_EU, SCRS_AFFT_WAVES_PER_EU))) void afft_pass_kernel(
+    AfftPassArgs a, uint32_t tpb) {
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
     constexpr uint32_t kTN = W ? kTabW : kTab;          // u16 entries per node table
     extern __shared__ __align__(256) uint32_t smem[];  // tables at multiples of 128 (wide: 256) bytes
@@ -254,7 +347,40 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
     auto node_tab = [&](const uint16_t *tabs, uint32_t beta, uint32_t low_row) {
         return tabs + ((G - (G >> beta)) + (low_row >> (beta + 1))) * kTN;
     };
+    // In a full stage (t == RB) register j holds local row
+    // (wv mod 2^s_lo) | (j << s_lo) | ((wv >> s_lo) << (s_lo + RB)), so the
+    // node of level s_lo + bs is the level's first node plus
+    // ((wv >> s_lo) << (RB - 1 - bs)) plus the compile-time j >> (bs + 1).
+    auto level_base = [&](const uint16_t *tabs, uint32_t s_lo, int bs) -> uint32_t {
+        const uint32_t beta = s_lo + static_cast<uint32_t>(bs);
+        const uint32_t node = (G - (G >> beta)) + ((wv >> s_lo) << (RB - 1 - static_cast<uint32_t>(bs)));
+        uint32_t lb = __builtin_amdgcn_readfirstlane(lds_byte(tabs) + node * (kTN * 2));
+        asm("" : "+s"(lb));
+        return lb;
+    };
     auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
+        if (t == RB) {
+#pragma unroll
+            for (int bs = 0; bs < static_cast<int>(RB); bs++) {
+                const uint32_t lb = level_base(inv_t, s_lo, bs);
+                // Butterfly n of the level: registers jn and jn + 2^bs.
+                auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
+                Parts<W> pp[kPipe + 1];
+#pragma unroll
+                for (int n = 0; n < R / 2 + kPipe; n++) {
+                    if (n < R / 2) {
+                        const int j = jn(n);
+                        v[j + (1 << bs)] ^= v[j];
+                        gmul2_issue<W>(pp[n % (kPipe + 1)], lb, static_cast<uint32_t>(j >> (bs + 1)) * (kTN * 2),
+                                       v[j + (1 << bs)]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (n >= kPipe) v[jn(n - kPipe)] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int bs = 0; bs < static_cast<int>(RB); bs++) {
             if (static_cast<uint32_t>(bs) >= t) break;
@@ -268,6 +394,30 @@ __global__ __launch_bounds__(1024) void afft_pass_kernel(AfftPassArgs a, uint32_
         }
     };
     auto fwd_stage = [&](uint32_t s_lo, uint32_t t) {
+        if (t == RB) {
+#pragma unroll
+            for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
+                const uint32_t lb = level_base(fwd_t, s_lo, bs);
+                auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
+                Parts<W> pp[kPipe + 1];
+#pragma unroll
+                for (int n = 0; n < R / 2 + kPipe; n++) {
+                    if (n < R / 2) {
+                        const int j = jn(n);
+                        gmul2_issue<W>(pp[n % (kPipe + 1)], lb, static_cast<uint32_t>(j >> (bs + 1)) * (kTN * 2),
+                                       v[j + (1 << bs)]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (n >= kPipe) {
+                        const int j = jn(n - kPipe);
+                        v[j] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
+                        v[j + (1 << bs)] ^= v[j];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
             if (static_cast<uint32_t>(bs) >= t) continue;
