@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "afft.hpp"
 #include "device.hpp"
@@ -347,40 +348,69 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     auto node_tab = [&](const uint16_t *tabs, uint32_t beta, uint32_t low_row) {
         return tabs + ((G - (G >> beta)) + (low_row >> (beta + 1))) * kTN;
     };
-    // In a full stage (t == RB) register j holds local row
-    // (wv mod 2^s_lo) | (j << s_lo) | ((wv >> s_lo) << (s_lo + RB)), so the
-    // node of level s_lo + bs is the level's first node plus
-    // ((wv >> s_lo) << (RB - 1 - bs)) plus the compile-time j >> (bs + 1).
-    auto level_base = [&](const uint16_t *tabs, uint32_t s_lo, int bs) -> uint32_t {
+    // In a stage over T bits at s_lo, T known at compile time, register j
+    // holds local row (u mod 2^s_lo) | ((j mod 2^T) << s_lo) | ((u >> s_lo) << (s_lo + T))
+    // with u = wv * (R >> T) + (j >> T).  The node of level s_lo + bs is the
+    // level's first node plus ((u >> s_lo) << (T - 1 - bs)) plus the
+    // compile-time ((j mod 2^T) >> (bs + 1)): one wave-uniform base per level
+    // and value of j >> T, the rest in the LDS instruction's offset.
+    auto level_base = [&](const uint16_t *tabs, uint32_t s_lo, int T, int bs, int uidx) -> uint32_t {
         const uint32_t beta = s_lo + static_cast<uint32_t>(bs);
-        const uint32_t node = (G - (G >> beta)) + ((wv >> s_lo) << (RB - 1 - static_cast<uint32_t>(bs)));
+        const uint32_t u = wv * static_cast<uint32_t>(R >> T) + static_cast<uint32_t>(uidx);
+        const uint32_t node = (G - (G >> beta)) + ((u >> s_lo) << (T - 1 - bs));
         uint32_t lb = __builtin_amdgcn_readfirstlane(lds_byte(tabs) + node * (kTN * 2));
         asm("" : "+s"(lb));
         return lb;
     };
-    auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
-        if (t == RB) {
+    // The butterflies of a level are software pipelined: the table reads of
+    // product n are issued before product n - kPipe is combined.
+    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic) {
+        constexpr int T = decltype(tc)::value;
+        constexpr bool inverse = decltype(ic)::value;
+        constexpr int NU = R >> T;
 #pragma unroll
-            for (int bs = 0; bs < static_cast<int>(RB); bs++) {
-                const uint32_t lb = level_base(inv_t, s_lo, bs);
-                // Butterfly n of the level: registers jn and jn + 2^bs.
-                auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
-                Parts<W> pp[kPipe + 1];
+        for (int step = 0; step < T; step++) {
+            const int bs = inverse ? step : T - 1 - step;
+            uint32_t lb[NU];
 #pragma unroll
-                for (int n = 0; n < R / 2 + kPipe; n++) {
-                    if (n < R / 2) {
-                        const int j = jn(n);
-                        v[j + (1 << bs)] ^= v[j];
-                        gmul2_issue<W>(pp[n % (kPipe + 1)], lb, static_cast<uint32_t>(j >> (bs + 1)) * (kTN * 2),
-                                       v[j + (1 << bs)]);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                    if (n >= kPipe) v[jn(n - kPipe)] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
-                    __builtin_amdgcn_sched_barrier(0);
+            for (int q = 0; q < NU; q++) lb[q] = level_base(tabs, s_lo, T, bs, q);
+            // Butterfly n of the level: registers jn and jn + 2^bs.
+            auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
+            auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kTN * 2); };
+            Parts<W> pp[kPipe + 1];
+#pragma unroll
+            for (int n = 0; n < R / 2 + kPipe; n++) {
+                if (n < R / 2) {
+                    const int j = jn(n);
+                    if constexpr (inverse) v[j + (1 << bs)] ^= v[j];
+                    gmul2_issue<W>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                if (n >= kPipe) {
+                    const int j = jn(n - kPipe);
+                    v[j] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
+                    if constexpr (!inverse) v[j + (1 << bs)] ^= v[j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            return;
         }
+    };
+    // Full stages, and 3-bit stages (a 7-level pass), take the fast path.
+    auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, auto ic) {
+        if (t == RB) {
+            stage_fast(tabs, s_lo, std::integral_constant<int, static_cast<int>(RB)>{}, ic);
+            return true;
+        }
+        if constexpr (GL == 0 && RB == 4) {
+            if (t == 3) {
+                stage_fast(tabs, s_lo, std::integral_constant<int, 3>{}, ic);
+                return true;
+            }
+        }
+        return false;
+    };
+    auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
+        if (fast_stage(inv_t, s_lo, t, std::true_type{})) return;
 #pragma unroll
         for (int bs = 0; bs < static_cast<int>(RB); bs++) {
             if (static_cast<uint32_t>(bs) >= t) break;
@@ -394,30 +424,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         }
     };
     auto fwd_stage = [&](uint32_t s_lo, uint32_t t) {
-        if (t == RB) {
-#pragma unroll
-            for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
-                const uint32_t lb = level_base(fwd_t, s_lo, bs);
-                auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
-                Parts<W> pp[kPipe + 1];
-#pragma unroll
-                for (int n = 0; n < R / 2 + kPipe; n++) {
-                    if (n < R / 2) {
-                        const int j = jn(n);
-                        gmul2_issue<W>(pp[n % (kPipe + 1)], lb, static_cast<uint32_t>(j >> (bs + 1)) * (kTN * 2),
-                                       v[j + (1 << bs)]);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                    if (n >= kPipe) {
-                        const int j = jn(n - kPipe);
-                        v[j] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
-                        v[j + (1 << bs)] ^= v[j];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-            return;
-        }
+        if (fast_stage(fwd_t, s_lo, t, std::false_type{})) return;
 #pragma unroll
         for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
             if (static_cast<uint32_t>(bs) >= t) continue;
@@ -745,8 +752,14 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     const bool tables = a.in_rows || a.copy_rows || a.out_rows;
     const bool a4 = a.L % 4 == 0 && a.L <= 0x7fffffffull && al4(a.in_base) && al4(a.out_base) && al4(a.aux_base) &&
                     (a.in_cw_stride | a.out_cw_stride | a.aux_cw_stride) % 4 == 0 && (!tables || a.rows_aligned4);
-    // Wide node tables for the 16-row passes whose tables still fit.
-    if (a4 && a.g >= 4 && a.node_tabs_w && lds_w <= 160 * 1024) {
+    // Wide node tables for the 16-row passes whose tables still fit, unless
+    // the nibble tables' smaller footprint lets more waves share a CU (at
+    // most 16 at these kernels' register use: 4 per SIMD).
+    auto waves_per_cu = [&](size_t bytes) {
+        const size_t wgs = (160u * 1024u) / bytes, waves = (size_t{1} << a.g) / 16;
+        return std::min<size_t>(16, wgs * waves);
+    };
+    if (a4 && a.g >= 4 && a.node_tabs_w && lds_w <= 160 * 1024 && waves_per_cu(lds_w) >= waves_per_cu(lds)) {
         if (a.g == 8) return launch_pass_r<16, true, 8, true>(a, D, count, lds_w, s);
         return launch_pass_r<16, true, 0, true>(a, D, count, lds_w, s);
     }
