@@ -31,6 +31,33 @@ HBM_PEAK_GBS = 8000.0
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
 # LDS read bandwidth: 256 CUs x 232 B/clk (ds_read_b128, measured) x 2.4 GHz.
 LDS_PEAK_TBS = 256 * 232 * 2.4e9 / 1e12
+# Scattered 2-byte table reads: an LDS unit serves 32 lanes per clock
+# without bank conflicts (SQ_LDS_IDX_ACTIVE = 2 cycles per wave64 read),
+# 256 CUs x 32 x 2.4 GHz.
+LDS_LOOKUP_PEAK_TPS = 256 * 32 * 2.4e9 / 1e12
+
+
+def afft_table_reads(k: int, m: int, sl: int) -> tuple[int, int]:
+    """Table reads (one 2-byte LDS read per lane) of the additive-FFT encode
+    and erasure decode at k = m = 2^D/2: a butterfly multiplies each 16-bit
+    element of its high row by the node constant, 3 reads with the 6+5+5-bit
+    tables of 8-level passes, 4 with the nibble tables of 7-level passes
+    (whose smaller footprint doubles the waves per CU); a per-row scale is 4
+    reads per element.  Encode: inverse over the 2^Dk data rows and forward
+    over the coset, in passes of (8, Dk-8) levels.  Decode over N = k + m
+    rows: inverse in two 8-level passes (input scale, then the T(q) scale),
+    the XOR-only derivative pass, the forward over the high bits (1/T scale)
+    and the forward over the low bits of the rows below k (output scale)."""
+    assert k == m and k & (k - 1) == 0 and 2 * k == 1 << 16
+    el = sl // 2
+    per = lambda g: 3 if g == 8 else 4  # noqa: E731
+    dk = k.bit_length() - 1
+    enc = (8 * per(8) + 2 * (dk - 8) * per(dk - 8) + 8 * per(8)) * (k // 2) * el
+    n = 2 * k
+    inv = 2 * 8 * per(8) * (n // 2) * el + 2 * n * el * 4
+    fwd_hi = 8 * per(8) * (n // 2) * el + n * el * 4
+    fwd_lo = 8 * per(8) * (k // 2) * el + k * el * 4
+    return enc, inv + fwd_hi + fwd_lo
 
 
 def erasure_indices(count: int, k: int, m: int, max_r: int, seed: int) -> np.ndarray:
@@ -252,6 +279,15 @@ def gf16_workload(args, which: str | None = None) -> dict:
     if which == "afft":
         targeted = afft_targeted_line(args, dec, data, rep, out, stream, k, m)
     lds_roof = None
+    if which == "afft":
+        enc_reads, dec_reads = afft_table_reads(k, m, sl)
+        enc_tps = enc_reads / (enc_ms / 1e3) / 1e12
+        dec_tps = dec_reads / (dec_ms / 1e3) / 1e12
+        lds_roof = {"kernel": "afft_pass_kernel, encode phase", "achieved": round(enc_tps, 3),
+                    "peak": round(LDS_LOOKUP_PEAK_TPS, 2), "unit": "T table reads/s",
+                    "frac": round(enc_tps / LDS_LOOKUP_PEAK_TPS, 4), "reads_per_launch": enc_reads,
+                    "decode": {"achieved": round(dec_tps, 3), "frac": round(dec_tps / LDS_LOOKUP_PEAK_TPS, 4),
+                               "reads_per_launch": dec_reads}}
     if which == "tower":
         # The feed reads, per element and term, four nibble-table entries of
         # each 16-row group: 2 bytes per row of the group, 8 bytes for groups
