@@ -175,7 +175,9 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
             return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
         cap = want;
     }
-    if ((err = d_tables.ensure(cap * tb)) != hipSuccess)
+    // HBM product tables only for the kernels that read them.
+    const bool hbm_tables = bitmatrix || write_tables;
+    if (hbm_tables && (err = d_tables.ensure(cap * tb)) != hipSuccess)
         return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
     DecodeWorkspace ws;
     ws.idx = d_idx.as<uint16_t>();
@@ -186,7 +188,7 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
     ws.src = d_src.as<const uint8_t *>();
     ws.dst = d_dst.as<uint8_t *>();
     ws.copy = d_copy.as<uint8_t *>();
-    ws.tables = d_tables.as<uint32_t>();
+    ws.tables = hbm_tables ? d_tables.as<uint32_t>() : nullptr;
     ws.cap = cap;
     for (uint64_t c0 = 0; c0 < count; c0 += cap) {
         const uint64_t cn = count - c0 < cap ? count - c0 : cap;
@@ -241,9 +243,10 @@ scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *co
                                     const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out, uint64_t cn,
                                     hipStream_t s, bool vec16, bool bitmatrix, bool may_allocate) {
     const uint64_t tb = cw_table_bytes(bitmatrix);
-    // GF(2^8) table path: the multiply-XOR builds its product tables in LDS
-    // from the fused coefficients, so none are written to HBM.
-    const bool fused_in_rows = !gf16 && !bitmatrix && !write_tables;
+    // Table path (either field): the multiply-XOR builds its product tables
+    // in LDS from the fused coefficients, so none are written to HBM, unless
+    // a recipe cache asked for them.
+    const bool fused_in_rows = !bitmatrix && !write_tables;
     if (!fused_in_rows && !ws.tables) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     if (cn > ws.cap) return st(SCRS_ERR_CAPACITY, ws.cap);
     hipError_t err;

@@ -11,6 +11,7 @@
 // costs four 32-byte LDS lookups and eight XOR3s for all 16 rows.
 #include <hip/hip_runtime.h>
 
+#include "gf_device.hpp"
 #include "kernels.hpp"
 #include "rows_device.hpp"
 
@@ -49,13 +50,62 @@ __device__ void pass16(const RowsTermsArgs &a, uint8_t *lds, uint64_t c, int rg,
     for (uint32_t t0 = tb; t0 < te; t0 += kChunk) {
         const uint32_t tc = min(kChunk, te - t0);
         __syncthreads();
-        // Stage: entry (q, n) of term t keeps its first EB bytes (rows 0..RW-1).
-        for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += kThreads) {
-            const uint32_t t = i / (64 * D), rem = i % (64 * D);
-            const uint32_t e = rem / D, d = rem % D;
-            const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
-            reinterpret_cast<uint32_t *>(lds)[t * 64 * D + lds16_index<EB>(e, d)] = *reinterpret_cast<const uint32_t *>(
-                tab_cw + (static_cast<uint64_t>(tt) * a.groups + rg) * kGroupBytes + e * 32 + d * 4);
+        if (DEC && a.fused) {
+            __builtin_amdgcn_sched_barrier(0);
+            // Build this chunk's tables from the fused coefficients instead
+            // of reading them: a thread takes one term, one row pair (dword
+            // d) and one nibble position q, forms the products of the two
+            // packed coefficients with the four basis elements of q, and
+            // writes the 16 entries of q as XORs of those.  For bits b < 8
+            // the basis element 2^b is the base-field x^b, which doubles
+            // both bytes of a + b u; for b >= 8 it is x^(b-8) u, and
+            // (a + b u) u = 0x20 b + (a + b) u (u^2 = u + 0x20).
+            const uint16_t *fc = a.fused + c * a.rmax * a.T;  // [row][term]
+            const uint32_t rows_c = load_uniform(a.rows + c);
+            // The thread's task indices are recomputed per chunk: hoisted out
+            // of the term loop they would stay live across the fold.
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            for (uint32_t i = tid; i < tc * D * 4; i += kThreads) {
+                const uint32_t q = i & 3u, d = (i >> 2) % D, t = (i >> 2) / D;
+                const uint32_t row = static_cast<uint32_t>(rg) * kRowsPerPass + 2 * d;
+                const uint64_t col = t0 + t;
+                const uint32_t f0 = row < rows_c ? fc[static_cast<uint64_t>(row) * a.T + col] : 0u;
+                const uint32_t f1 = row + 1 < rows_c ? fc[static_cast<uint64_t>(row + 1) * a.T + col] : 0u;
+                uint32_t v = f0 | (f1 << 16);
+                if (q >= 2) {
+                    const uint32_t lo = v & 0x00ff00ffu, hi = (v >> 8) & 0x00ff00ffu;
+                    uint32_t hi20 = hi;
+#pragma unroll
+                    for (int x = 0; x < 5; x++) hi20 = gf8_xtime4(hi20);
+                    v = hi20 | ((lo ^ hi) << 8);
+                }
+                if (q & 1u) v = gf8_xtime4(gf8_xtime4(gf8_xtime4(gf8_xtime4(v))));
+                uint32_t b[4];
+                b[0] = v;
+#pragma unroll
+                for (int x = 1; x < 4; x++) b[x] = gf8_xtime4(b[x - 1]);
+                uint32_t *dst = reinterpret_cast<uint32_t *>(lds) + t * 64 * D;
+                // Gray-code order: each next entry differs by one basis term.
+                uint32_t e = 0;
+                dst[lds16_index<EB>(q * 16, d)] = 0;
+#pragma unroll
+                for (int g = 1; g < 16; g++) {
+                    e ^= b[__builtin_ctz(g)];
+                    dst[lds16_index<EB>(q * 16 + (g ^ (g >> 1)), d)] = e;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            // Stage: entry (q, n) of term t keeps its first EB bytes (rows 0..RW-1).
+            for (uint32_t i = threadIdx.x; i < tc * 64 * D; i += kThreads) {
+                const uint32_t t = i / (64 * D), rem = i % (64 * D);
+                const uint32_t e = rem / D, d = rem % D;
+                const uint32_t tt = a.term_index ? a.term_index[t0 + t] : t0 + t;
+                reinterpret_cast<uint32_t *>(lds)[t * 64 * D + lds16_index<EB>(e, d)] =
+                    *reinterpret_cast<const uint32_t *>(tab_cw + (static_cast<uint64_t>(tt) * a.groups + rg) * kGroupBytes +
+                                                        e * 32 + d * 4);
+            }
         }
         __syncthreads();
         if (!active) continue;

@@ -36,9 +36,9 @@ struct RowsTermsArgs {
     const uint32_t *term_index = nullptr; // table term of source t (device, [T]), or null: t
     uint64_t L = 0;                   // symbol length in bytes
     uint64_t count = 0;               // codewords
-    // Decode only, GF(2^8): the fused coefficients ([codeword][rmax rows]
-    // [T terms], u16) from which the kernel builds its product tables in
-    // LDS; `tables` is then unused.
+    // Decode only (either field): the fused coefficients ([codeword][rmax
+    // rows][T terms], u16) from which the kernel builds its product tables
+    // in LDS; `tables` is then unused.
     const uint16_t *fused = nullptr;
     // Set by the caller: every destination row address and L are multiples
     // of 4, so a launch too small to fill the device may split each
