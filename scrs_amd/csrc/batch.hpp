@@ -34,9 +34,24 @@ struct BatchRange {
     uint32_t cw_count = 0, ncw = 1;       // codewords, codewords per unit
     uint32_t unit_base = 0;               // first unit of the range
     uint32_t flags = 0;
-    // Term segments: with segs > 1 each unit folds seg_terms terms and XORs
-    // its partial rows into destinations zeroed beforehand (16-byte rows).
+    // Term segments: with segs > 1 each unit folds seg_terms terms and
+    // stores its partial rows to partials[codeword][row][segment], rows of
+    // part_stride bytes (L rounded up to 16); a reduce launch XORs them into
+    // the destinations.
     uint32_t segs = 1, seg_terms = 0;
+    uint8_t *partials = nullptr;
+    uint64_t part_stride = 0;
+};
+
+// The reduce of one term-split range: destination row (c, row) is the XOR
+// of its segments' partial rows, one dword per item.  Items of the launch
+// are numbered range after range from item_base.
+struct BatchReduce {
+    uint8_t *const *dst = nullptr;    // the range's cw_count * rows destination rows
+    const uint8_t *partials = nullptr;
+    uint64_t L = 0, stride = 0;
+    uint64_t item_base = 0;
+    uint32_t rows = 0, segs = 0;
 };
 
 // A table job: either the closed-form solve of one decode recipe (fused
@@ -56,25 +71,40 @@ struct BatchSolve {
 };
 
 struct BatchHeader {
-    uint32_t nranges = 0, nunits = 0, nsolve = 0, nzero = 0;
+    uint32_t nranges = 0, nunits = 0, nsolve = 0, nred = 0;
+    // Value of the unit counter when the launch starts (see launch_batch_rows).
+    uint32_t claim_base = 0, pad = 0;
+    uint64_t red_items = 0;            // dword items of the reduce launch
     const BatchRange *ranges = nullptr;
     const BatchSolve *solves = nullptr;
-    const uint64_t *zero = nullptr;  // nzero (address, bytes) pairs cleared before the multiply-XOR
+    const BatchReduce *red = nullptr;  // nred term-split ranges, in item order
 };
 
 // Tile widths the multiply-XOR kernel is compiled for (16 bytes per thread).
 constexpr uint32_t kBatchTileBytes[3] = {1024, 2048, 4096};
 
 // The launches read every size and array address from the device header, so
-// their arguments stay fixed across batches.  The multiply-XOR launch takes
-// units from counter[0] and leaves counter[0..1] at zero when it ends (zero
-// before the first launch), so no memset sits between launches and a batch
+// their arguments stay fixed across batches.  The multiply-XOR launch is
+// persistent: workgroup b first runs unit b, then draws units from
+// counter[0], which only ever grows.  A launch draws exactly `nunits` times
+// in all (each workgroup that ran a unit draws once past the end), so the
+// caller passes the counter's value at the start as hdr->claim_base and
+// adds nunits afterwards: no reset, no memset between launches, and a batch
 // with nothing to solve runs the multiply-XOR alone.
 hipError_t launch_batch_solve(const BatchHeader *hdr, const uint8_t *gf8_tables, const uint16_t *gf16_log_exp,
                               uint32_t grid, hipStream_t s);
 hipError_t launch_batch_rows(const BatchHeader *hdr, uint32_t *counter, uint32_t tile_bytes, uint32_t grid,
                              hipStream_t s);
+// XORs the partial rows of the term-split ranges into their destinations.
+hipError_t launch_batch_reduce(const BatchHeader *hdr, uint32_t grid, hipStream_t s);
 // Persistent grid that fills the device for the given tile width.
 uint32_t batch_rows_grid(uint32_t tile_bytes);
+// Grid of the reduce launch: enough workgroups for `items` dwords, at most
+// kBatchReduceGrid (grid-stride beyond).
+constexpr uint32_t kBatchReduceGrid = 2048;
+inline uint32_t batch_reduce_grid(uint64_t items) {
+    const uint64_t g = (items + 255) / 256;
+    return static_cast<uint32_t>(g < 1 ? 1 : (g > kBatchReduceGrid ? kBatchReduceGrid : g));
+}
 
 }  // namespace scrs

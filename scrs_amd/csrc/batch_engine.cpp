@@ -58,6 +58,7 @@ struct BatchEngine::Plan {
     // addresses at upload.
     struct RangeRef {
         uint64_t src = 0, dst = 0, copy = 0;
+        uint64_t part = 0;  // byte offset of a term-split range's partial rows
         bool has_copy = false;
     };
     struct SolveRef {
@@ -65,7 +66,9 @@ struct BatchEngine::Plan {
     };
     std::vector<RangeRef> rref;
     std::vector<SolveRef> sref;
-    std::vector<uint64_t> zero;   // (address, bytes) of rows cleared before term-split ranges
+    std::vector<BatchReduce> red;    // term-split ranges (dst and partials patched at upload)
+    std::vector<uint32_t> red_range; // the range of each
+    uint64_t part_bytes = 0, red_items = 0;
     uint64_t work_u32 = 0, fused_u16 = 0;
     uint32_t units = 0;
 };
@@ -119,7 +122,7 @@ scrs_status BatchEngine::init(const scrs_batch_config &cfg) {
     for (Slot &s : slots_) {
         if (err == hipSuccess) err = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         if (err == hipSuccess) err = s.counter.ensure(64);
-        if (err == hipSuccess) err = hipMemset(s.counter.ptr, 0, 64);  // the kernel keeps it at zero after
+        if (err == hipSuccess) err = hipMemset(s.counter.ptr, 0, 64);  // launches count on from claims = 0
     }
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }
@@ -456,11 +459,11 @@ scrs_status BatchEngine::plan(const scrs_job *jobs, uint64_t count, uint32_t til
 }
 
 // A batch too small to give every CU work splits the term loop of its
-// ranges with dword-aligned destination rows: each segment's unit XORs its
-// partial rows into destinations the solve launch clears first.  A unit's
-// term loop waits out one memory latency per term, so the latency of a lone
-// codeword drops with the segment length (kMinSegTerms at the shortest); a
-// full batch is left as it is.
+// ranges with dword-aligned destination rows: each segment's unit stores its
+// partial rows, and a reduce launch XORs them into the destinations.  A
+// unit's term loop waits out one memory latency per term, so the latency of
+// a lone codeword drops with the segment length (kMinSegTerms at the
+// shortest); a full batch is left as it is.
 void BatchEngine::split_terms(Plan &p, uint64_t &units) {
     if (units >= kFillUnits || units == 0) return;
     const uint64_t factor = (kFillUnits + units - 1) / units;
@@ -479,11 +482,19 @@ void BatchEngine::split_terms(Plan &p, uint64_t &units) {
             continue;
         }
         any = true;
-        const Plan::RangeRef &ref = p.rref[i];
-        for (uint64_t x = 0; x < static_cast<uint64_t>(r.cw_count) * r.rows; x++) {
-            p.zero.push_back(p.ptrs[ref.dst + x]);
-            p.zero.push_back(r.L);
-        }
+        r.part_stride = align16(r.L);
+        p.rref[i].part = p.part_bytes;
+        const uint64_t out_rows = static_cast<uint64_t>(r.cw_count) * r.rows;
+        p.part_bytes += out_rows * r.segs * r.part_stride;
+        BatchReduce rd;
+        rd.L = r.L;
+        rd.stride = r.part_stride;
+        rd.item_base = p.red_items;
+        rd.rows = r.rows;
+        rd.segs = r.segs;
+        p.red.push_back(rd);
+        p.red_range.push_back(static_cast<uint32_t>(i));
+        p.red_items += out_rows * (r.L / 4);
     }
     if (!any) return;
     units = 0;
@@ -493,8 +504,9 @@ void BatchEngine::split_terms(Plan &p, uint64_t &units) {
     }
 }
 
-hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
-    auto it = slot.graphs.find(tile);
+hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, bool reduce, hipStream_t s) {
+    const uint32_t key = tile * 2 + (reduce ? 1 : 0);
+    auto it = slot.graphs.find(key);
     if (it == slot.graphs.end() || it->second.first != slot.generation) {
         if (it != slot.graphs.end()) {
             (void)hipGraphExecDestroy(it->second.second);
@@ -505,14 +517,16 @@ hipError_t BatchEngine::run_graph(Slot &slot, uint32_t tile, hipStream_t s) {
         hipGraphExec_t exec = nullptr;
         hipError_t err = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal);
         if (err != hipSuccess) return err;
-        // The kernel returns its unit counter to zero itself: no memset node.
+        // The unit counter needs no reset between launches: no memset node.
+        // The reduce launch is sized for the largest reduce (grid-stride).
         hipError_t e3 = launch_batch_rows(hdr, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), capture_);
+        if (e3 == hipSuccess && reduce) e3 = launch_batch_reduce(hdr, kBatchReduceGrid, capture_);
         err = hipStreamEndCapture(capture_, &graph);
         if (err == hipSuccess) err = e3;
         if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (graph) (void)hipGraphDestroy(graph);
         if (err != hipSuccess) return err;
-        it = slot.graphs.emplace(tile, std::make_pair(slot.generation, exec)).first;
+        it = slot.graphs.emplace(key, std::make_pair(slot.generation, exec)).first;
     }
     return hipGraphLaunch(it->second.second, s);
 }
@@ -522,10 +536,11 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     const uint64_t off_s = align16(off_r + p.ranges.size() * sizeof(BatchRange));
     const uint64_t off_p = align16(off_s + p.solves.size() * sizeof(BatchSolve));
     const uint64_t off_u = align16(off_p + p.ptrs.size() * 8);
-    const uint64_t off_z = align16(off_u + p.u16s.size() * 2);
-    const uint64_t total = align16(off_z + p.zero.size() * 8);
+    const uint64_t off_red = align16(off_u + p.u16s.size() * 2);
+    const uint64_t total = align16(off_red + p.red.size() * sizeof(BatchReduce));
     const uint64_t sc_work = 0, sc_fused = align16(p.work_u32 * 4);
-    const uint64_t sc_total = std::max<uint64_t>(16, align16(sc_fused + p.fused_u16 * 2));
+    const uint64_t sc_part = align16(sc_fused + p.fused_u16 * 2);
+    const uint64_t sc_total = std::max<uint64_t>(16, align16(sc_part + p.part_bytes));
     hipError_t err;
     // The slot's previous batch may still read its staging and arrays.
     if ((err = hipEventSynchronize(slot.done)) != hipSuccess) return err;
@@ -545,8 +560,10 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     hdr.nsolve = static_cast<uint32_t>(p.solves.size());
     hdr.ranges = reinterpret_cast<const BatchRange *>(d + off_r);
     hdr.solves = reinterpret_cast<const BatchSolve *>(d + off_s);
-    hdr.nzero = static_cast<uint32_t>(p.zero.size() / 2);
-    hdr.zero = reinterpret_cast<const uint64_t *>(d + off_z);
+    hdr.nred = static_cast<uint32_t>(p.red.size());
+    hdr.red_items = p.red_items;
+    hdr.red = reinterpret_cast<const BatchReduce *>(d + off_red);
+    hdr.claim_base = slot.claims;
     std::memcpy(h, &hdr, sizeof(hdr));
     auto *hr = reinterpret_cast<BatchRange *>(h + off_r);
     for (size_t i = 0; i < p.ranges.size(); i++) {
@@ -555,7 +572,15 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
         r.src = reinterpret_cast<const uint8_t *const *>(d + off_p + ref.src * 8);
         r.dst = reinterpret_cast<uint8_t *const *>(d + off_p + ref.dst * 8);
         r.copy = ref.has_copy ? reinterpret_cast<uint8_t *const *>(d + off_p + ref.copy * 8) : nullptr;
+        r.partials = r.segs > 1 ? sc + sc_part + ref.part : nullptr;
         hr[i] = r;
+    }
+    auto *hred = reinterpret_cast<BatchReduce *>(h + off_red);
+    for (size_t i = 0; i < p.red.size(); i++) {
+        BatchReduce rd = p.red[i];
+        rd.dst = hr[p.red_range[i]].dst;
+        rd.partials = hr[p.red_range[i]].partials;
+        hred[i] = rd;
     }
     auto *hs = reinterpret_cast<BatchSolve *>(h + off_s);
     for (size_t i = 0; i < p.solves.size(); i++) {
@@ -574,22 +599,30 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     }
     if (!p.ptrs.empty()) std::memcpy(h + off_p, p.ptrs.data(), p.ptrs.size() * 8);
     if (!p.u16s.empty()) std::memcpy(h + off_u, p.u16s.data(), p.u16s.size() * 2);
-    if (!p.zero.empty()) std::memcpy(h + off_z, p.zero.data(), p.zero.size() * 8);
-    // Tables and recipes written by the previous batch's solve may be read
-    // here, and it may have run on another stream.  Only its solve is
-    // waited for: its multiply-XOR may still run beside this batch.
-    if ((err = hipStreamWaitEvent(s, last_solved_, 0)) != hipSuccess) return err;
+    // Tables and recipes written by an earlier batch's solve may be read
+    // here, and it may have run on another stream.  Only the latest solve is
+    // waited for: the multiply-XORs may still run beside this batch.
+    if (solves_recorded_ && (err = hipStreamWaitEvent(s, last_solved_, 0)) != hipSuccess) return err;
     if ((err = hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s)) != hipSuccess) return err;
     const BatchHeader *dh = slot.dev.as<BatchHeader>();
-    // Nothing to solve and no split destinations to clear (every table and
-    // recipe resident): the multiply-XOR runs alone.
-    if ((!p.solves.empty() || !p.zero.empty()) &&
-        (err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, s)) != hipSuccess)
-        return err;
-    if ((err = hipEventRecord(last_solved_, s)) != hipSuccess) return err;
-    err = cfg_.use_graph ? run_graph(slot, tile, s)
-                         : launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, batch_rows_grid(tile), s);
+    // Nothing to solve (every table and recipe resident): the multiply-XOR
+    // runs alone, and the latest solve stays the one to wait for.
+    if (!p.solves.empty()) {
+        if ((err = launch_batch_solve(dh, gf8_.as<uint8_t>(), gf16_log_exp_, kSolveGrid, s)) != hipSuccess)
+            return err;
+        if ((err = hipEventRecord(last_solved_, s)) != hipSuccess) return err;
+        solves_recorded_ = true;
+    }
+    const bool reduce = !p.red.empty();
+    if (cfg_.use_graph) {
+        err = run_graph(slot, tile, reduce, s);
+    } else {
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(batch_rows_grid(tile), p.units));
+        err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, grid, s);
+        if (err == hipSuccess && reduce) err = launch_batch_reduce(dh, batch_reduce_grid(p.red_items), s);
+    }
     if (err != hipSuccess) return err;
+    slot.claims += p.units;
     slot.seq = seq_;
     return hipEventRecord(slot.done, s);
 }

@@ -78,12 +78,13 @@ class BatchEngine {
         HostBuf host;        // staged header, ranges, solves, pointer and index arrays
         DevBuf dev;          // device copy of the staged bytes
         DevBuf scratch;      // solve work and fused coefficients
-        DevBuf counter;      // unit queue (zero between launches)
+        DevBuf counter;      // unit queue: draws only ever add to it
+        uint32_t claims = 0; // its value when the next launch starts
         hipEvent_t done = nullptr;
         uint64_t seq = 0;
         uint32_t generation = 0;  // bumps when `dev` moves
         // Multiply-XOR launches captured against this slot's buffers, per
-        // tile width.
+        // tile width and with or without the reduce launch.
         std::map<uint32_t, std::pair<uint32_t, hipGraphExec_t>> graphs;
     };
     // Staging slots: a batch reuses the slot of the batch four back, so
@@ -120,7 +121,7 @@ class BatchEngine {
     scrs_status plan(const scrs_job *jobs, uint64_t count, uint32_t tile, uint32_t cw, Plan &p);
     void split_terms(Plan &p, uint64_t &units);
     hipError_t launch(Slot &slot, const Plan &p, uint32_t tile, hipStream_t s);
-    hipError_t run_graph(Slot &slot, uint32_t tile, hipStream_t s);
+    hipError_t run_graph(Slot &slot, uint32_t tile, bool reduce, hipStream_t s);
     CacheEntry *encoder_tables(const scrs_profile &p, Plan &plan, scrs_status *err);
     CacheEntry *new_entry(uint64_t bytes, scrs_status *err);
     bool make_room();
@@ -142,6 +143,7 @@ class BatchEngine {
     // The previous batch's tables and recipes are written (its solve has
     // run), on whatever stream: a batch's multiply-XOR may read them.
     hipEvent_t last_solved_ = nullptr;
+    bool solves_recorded_ = false;
     // Resident caches, least recently used first.
     std::map<std::tuple<int32_t, uint64_t, uint64_t>, CacheEntry *> enc_cache_;
     std::unordered_map<std::vector<uint64_t>, CacheEntry *, RecipeKeyHash> recipe_cache_;

@@ -163,29 +163,35 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(const BatchHeader *hdr
         }
         __syncthreads();
     }
-    // Destinations of term-split ranges start from zero.
-    for (uint32_t z = blockIdx.x; z < hdr->nzero; z += gridDim.x) {
-        const uint64_t addr = hdr->zero[2 * z], len = hdr->zero[2 * z + 1];
-        if (((addr | len) & 15) == 0) {
-            SCRS_GLOBAL v4u32 *dst = gbl_mut<v4u32>(reinterpret_cast<void *>(addr));
-            for (uint64_t i = threadIdx.x; i < len / 16; i += blockDim.x) dst[i] = v4u32{0, 0, 0, 0};
-        } else {  // dword-aligned rows, the only other kind a split range has
-            SCRS_GLOBAL uint32_t *dst = gbl_mut<uint32_t>(reinterpret_cast<void *>(addr));
-            for (uint64_t i = threadIdx.x; i < len / 4; i += blockDim.x) dst[i] = 0;
-        }
-    }
 }
 
 // ---- multiply-XOR ----------------------------------------------------------
 
+// Row r of codeword c's row group rg: its destination row, or the unit's
+// own partial row when the range is term-split.
+__device__ __forceinline__ uint8_t *partial_row(const BatchRange &R, uint32_t c, uint32_t row, uint32_t seg) {
+    return R.partials + ((static_cast<uint64_t>(c) * R.rows + row) * R.segs + seg) * R.part_stride;
+}
+
+// 16 bytes of an output row at column p < L: the destination row, or the
+// segment's partial row (16-byte aligned, part_stride >= L rounded up to 16).
+template <bool VEC>
+__device__ __forceinline__ void store_out(const BatchRange &R, uint32_t c, uint32_t row, uint32_t seg, uint64_t p,
+                                          const uint32_t (&o)[4]) {
+    if (R.segs > 1) {
+        *gbl_mut<v4u32>(partial_row(R, c, row, seg) + p) = v4u32{o[0], o[1], o[2], o[3]};
+    } else {
+        store16<VEC>(load_uniform(R.dst + c * R.rows + row), p, R.L, o);
+    }
+}
+
 template <bool VEC, bool DEC, int RW, int TPU>
 __device__ __forceinline__ void unit8(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here, uint64_t p,
-                      uint32_t tb, uint32_t te, uint8_t *lds) {
+                      uint32_t tb, uint32_t te, uint32_t seg, uint8_t *lds) {
     constexpr uint32_t D = RW / 4;
     const bool active = p < R.L;
     const bool single = te - tb <= kChunk8;
     const bool do_copy = DEC && rg == 0;
-    const bool split = R.segs > 1;  // rows are dword-aligned (planner)
     for (uint32_t c = cw0; c < cw0 + ncw; c++) {
         uint32_t acc[16][D];
 #pragma unroll
@@ -231,28 +237,25 @@ __device__ __forceinline__ void unit8(const BatchRange &R, uint32_t cw0, uint32_
 #pragma unroll
         for (int r = 0; r < RW; r++) {
             if (static_cast<uint32_t>(r) >= rows_here) continue;
-            uint8_t *dst = load_uniform(R.dst + c * R.rows + rg * 16 + r);
             const int d = r >> 2, s = (r & 3) * 8;
             uint32_t o[4];
 #pragma unroll
             for (int jj = 0; jj < 4; jj++)
                 o[jj] = ((acc[4 * jj + 0][d] >> s) & 0xffu) | (((acc[4 * jj + 1][d] >> s) & 0xffu) << 8) |
                         (((acc[4 * jj + 2][d] >> s) & 0xffu) << 16) | (((acc[4 * jj + 3][d] >> s) & 0xffu) << 24);
-            if (split) xor16_atomic(dst, p, R.L, o);
-            else store16<VEC>(dst, p, R.L, o);
+            store_out<VEC>(R, c, rg * 16 + r, seg, p, o);
         }
     }
 }
 
 template <bool VEC, bool DEC, int RW, int TPU>
 __device__ __forceinline__ void unit16(const BatchRange &R, uint32_t cw0, uint32_t ncw, uint32_t rg, uint32_t rows_here,
-                       uint64_t p, uint32_t tb, uint32_t te, uint8_t *lds) {
+                       uint64_t p, uint32_t tb, uint32_t te, uint32_t seg, uint8_t *lds) {
     constexpr int EB = 2 * RW;
     constexpr uint32_t D = EB / 4;
     const bool active = p < R.L;
     const bool single = te - tb <= kChunk16;
     const bool do_copy = DEC && rg == 0;
-    const bool split = R.segs > 1;  // rows are dword-aligned (planner)
     for (uint32_t c = cw0; c < cw0 + ncw; c++) {
         uint32_t acc[8][D];
 #pragma unroll
@@ -296,14 +299,12 @@ __device__ __forceinline__ void unit16(const BatchRange &R, uint32_t cw0, uint32
 #pragma unroll
         for (int r = 0; r < RW; r++) {
             if (static_cast<uint32_t>(r) >= rows_here) continue;
-            uint8_t *dst = load_uniform(R.dst + c * R.rows + rg * 16 + r);
             const int d = r >> 1, sh = (r & 1) * 16;
             uint32_t o[4];
 #pragma unroll
             for (int jj = 0; jj < 4; jj++)
                 o[jj] = ((acc[2 * jj][d] >> sh) & 0xffffu) | (((acc[2 * jj + 1][d] >> sh) & 0xffffu) << 16);
-            if (split) xor16_atomic(dst, p, R.L, o);
-            else store16<VEC>(dst, p, R.L, o);
+            store_out<VEC>(R, c, rg * 16 + r, seg, p, o);
         }
     }
 }
@@ -333,13 +334,13 @@ __device__ __forceinline__ void run_unit(const BatchRange &R, uint32_t cw0, uint
         return;
     }
     if (R.flags & kRangeGf16) {
-        if (rows_here <= 4) unit16<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
-        else if (rows_here <= 8) unit16<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
-        else unit16<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
+        if (rows_here <= 4) unit16<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, seg, lds);
+        else if (rows_here <= 8) unit16<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, seg, lds);
+        else unit16<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, seg, lds);
     } else {
-        if (rows_here <= 4) unit8<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
-        else if (rows_here <= 8) unit8<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
-        else unit8<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, lds);
+        if (rows_here <= 4) unit8<VEC, DEC, 4, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, seg, lds);
+        else if (rows_here <= 8) unit8<VEC, DEC, 8, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, seg, lds);
+        else unit8<VEC, DEC, 16, TPU>(R, cw0, ncw, rg, rows_here, p, tb, te, seg, lds);
     }
 }
 
@@ -355,27 +356,17 @@ __device__ __forceinline__ BatchRange load_range(const BatchRange *p) {
     return r;
 }
 
-// counter[0] hands out units; counter[1] counts the workgroups that have
-// left the loop.  The last one to leave returns both to zero for the next
-// launch: every other workgroup has already drawn its final unit, so nothing
-// reads the counter after that.
+// Workgroup b runs unit b, then draws units from counter[0] (see
+// launch_batch_rows): a workgroup without a first unit draws nothing, every
+// other one draws until it is past the end.
 template <int TPU>
 __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr, uint32_t *counter) {
     __shared__ __align__(16) uint8_t lds[kLds];
     __shared__ uint32_t s_unit;
-    const uint32_t nunits = hdr->nunits, nranges = hdr->nranges;
+    const uint32_t nunits = hdr->nunits, nranges = hdr->nranges, base = hdr->claim_base;
     const BatchRange *ranges = hdr->ranges;
-    for (;;) {
-        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t u = __builtin_amdgcn_readfirstlane(s_unit);
-        if (u >= nunits) {
-            if (threadIdx.x == 0 && atomicAdd(counter + 1, 1u) == gridDim.x - 1) {
-                __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(counter + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            break;
-        }
+    uint32_t u = blockIdx.x;
+    while (u < nunits) {
         // Last range whose first unit is <= u.
         uint32_t lo = 0, hi = nranges - 1;
         while (lo < hi) {
@@ -398,6 +389,35 @@ __global__ __launch_bounds__(TPU) void batch_rows_kernel(const BatchHeader *hdr,
         else if (dec) run_unit<false, true, TPU>(R, cw0, ncw, rg, seg, p, lds);
         else run_unit<false, false, TPU>(R, cw0, ncw, rg, seg, p, lds);
         __syncthreads();
+        // Draws past the first unit; the counter wraps, the difference does not.
+        if (threadIdx.x == 0) s_unit = gridDim.x + (atomicAdd(counter, 1u) - base);
+        __syncthreads();
+        u = __builtin_amdgcn_readfirstlane(s_unit);
+    }
+}
+
+// Reduce of the term-split ranges: item i is one dword of one destination
+// row, the XOR of that dword over the row's segment partials.
+__global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchHeader *hdr) {
+    const uint32_t nred = hdr->nred;
+    const uint64_t items = hdr->red_items;
+    const BatchReduce *red = hdr->red;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < items;
+         i += static_cast<uint64_t>(gridDim.x) * 256) {
+        uint32_t lo = 0, hi = nred - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (red[mid].item_base <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchReduce &d = red[lo];
+        const uint64_t words = d.L / 4, local = i - d.item_base;
+        const uint64_t row = local / words, w = local - row * words;  // row = codeword * rows + row in codeword
+        const uint8_t *src = d.partials + row * d.segs * d.stride + 4 * w;
+        uint32_t v = 0;
+#pragma unroll 4
+        for (uint32_t sg = 0; sg < d.segs; sg++) v ^= *gbl<uint32_t>(src + sg * d.stride);
+        *gbl_mut<uint32_t>(d.dst[row] + 4 * w) = v;
     }
 }
 
@@ -413,6 +433,11 @@ uint32_t batch_rows_grid(uint32_t tile_bytes) {
     // 256 CUs; LDS (32 KiB + a word) bounds residency at four workgroups.
     (void)tile_bytes;
     return 256 * 4;
+}
+
+hipError_t launch_batch_reduce(const BatchHeader *hdr, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(grid), dim3(256), 0, s, hdr);
+    return hipGetLastError();
 }
 
 hipError_t launch_batch_rows(const BatchHeader *hdr, uint32_t *counter, uint32_t tile_bytes, uint32_t grid,
