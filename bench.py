@@ -29,6 +29,7 @@ SEED_DATA = 0x5C25_0002
 SEED_ERASE = 0x5C25_E7A5
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
+FP4_PEAK_FLOPS = 10.1e15  # MI355X dense fp4 / fp6 MFMA
 # LDS read bandwidth: 256 CUs x 232 B/clk (ds_read_b128, measured) x 2.4 GHz.
 LDS_PEAK_TBS = 256 * 232 * 2.4e9 / 1e12
 # Scattered 2-byte table reads: an LDS unit serves 32 lanes per clock
@@ -893,14 +894,13 @@ def gf256_line(args) -> dict:
     ms_per_step = wall * 1e3 / args.steps
     value = (enc_bytes + dec_bytes) / (wall / args.steps) / 1e9
 
-    # The second GF(256) kernel on the same workload: the bit-matrix product
-    # on int8 MFMA, timed the same way and checked against the table path.
-    bitmatrix = None
-    if not args.no_bitmatrix:
-        from scrs_amd import Kernel
+    # The other GF(256) kernels on the same workload: the bit-matrix product
+    # on int8 MFMA and on e2m1 (fp4) MFMA, timed the same way and checked
+    # against the table path.
+    def mfma_leg(kind, chunk, peak, label):
         benc, bdec = batch_encoder(prof), batch_decoder(prof)
-        benc.set_kernel(Kernel.BITMATRIX)
-        bdec.set_kernel(Kernel.BITMATRIX)
+        benc.set_kernel(kind)
+        bdec.set_kernel(kind)
         rep_b = torch.empty_like(rep)
         out_b = torch.empty_like(out)
         for _ in range(max(1, args.warmup)):
@@ -917,22 +917,28 @@ def gf256_line(args) -> dict:
             e2.record(stream)
         torch.cuda.synchronize()
         if not torch.equal(rep_b, rep) or not torch.equal(out_b, data):
-            raise SystemExit("bit-matrix kernel output differs from the table kernel")
+            raise SystemExit(f"{kind.name} kernel output differs from the table kernel")
         b_enc = sum(a.elapsed_time(b) for a, b, _ in evb) / args.steps
         b_dec = sum(b.elapsed_time(c) for _, b, c in evb) / args.steps
-        # int8 MFMA work per launch: 2 * (8 rows-bits) * (8 source-bits) per
-        # byte column, with rows padded to 16 per group and sources to 8 per chunk.
+        # MFMA work per launch: 2 * (8 row bits) * (8 source bits) per byte
+        # column, rows padded to 16 per group and sources to whole chunks.
         groups = (M + 15) // 16
-        k_pad = (K + 7) // 8 * 8
+        k_pad = (K + chunk - 1) // chunk * chunk
         ops = 2.0 * (8 * 16 * groups) * (8 * k_pad) * L * count
-        bitmatrix = {
+        del rep_b, out_b
+        return {
             "encode_ms": round(b_enc, 4), "decode_ms": round(b_dec, 4),
             "encode_gbs": round(enc_bytes / (b_enc / 1e3) / 1e9, 2),
             "decode_gbs": round(dec_bytes / (b_dec / 1e3) / 1e9, 2),
-            "encode_int8_tops": round(ops / (b_enc / 1e3) / 1e12, 1),
-            "mfma_frac_of_int8_peak": round(ops / (b_enc / 1e3) / INT8_PEAK_OPS, 4),
+            f"encode_{label}_tops": round(ops / (b_enc / 1e3) / 1e12, 1),
+            f"mfma_frac_of_{label}_peak": round(ops / (b_enc / 1e3) / peak, 4),
         }
-        del rep_b, out_b
+
+    bitmatrix = bitmatrix_fp4 = None
+    if not args.no_bitmatrix:
+        from scrs_amd import Kernel
+        bitmatrix = mfma_leg(Kernel.BITMATRIX, 8, INT8_PEAK_OPS, "int8")
+        bitmatrix_fp4 = mfma_leg(Kernel.BITMATRIX_FP4, 16, FP4_PEAK_FLOPS, "fp4")
 
     cpu = None
     if not args.no_cpu:
@@ -1000,6 +1006,7 @@ def gf256_line(args) -> dict:
         },
         "cpu_baseline": cpu,
         "bitmatrix_kernel": bitmatrix,
+        "bitmatrix_fp4_kernel": bitmatrix_fp4,
         "host_slice": host,
     }
     del data, rep, out

@@ -151,14 +151,17 @@ scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, const uint8_t *
 /* ---- Multiply-XOR kernel of the GF(256) codecs (no reference counterpart) */
 
 typedef enum scrs_kernel {
-    SCRS_KERNEL_TABLE = 0,    /* split-nibble product tables in LDS (default) */
-    SCRS_KERNEL_BITMATRIX = 1 /* GF(2) bit-matrix product on int8 MFMA */
+    SCRS_KERNEL_TABLE = 0,         /* split-nibble product tables in LDS (default) */
+    SCRS_KERNEL_BITMATRIX = 1,     /* GF(2) bit-matrix product on int8 MFMA */
+    SCRS_KERNEL_BITMATRIX_FP4 = 2  /* the same product on e2m1 (fp4) MFMA, fp32 accumulation */
 } scrs_kernel;
 
-/* Select the multiply-XOR kernel of a GF(256) batch encoder or decoder.  Both
+/* Select the multiply-XOR kernel of a GF(256) batch encoder or decoder.  All
  * kernels produce the same bytes.  BITMATRIX expresses the code as an
  * (8m) x (8k) 0/1 matrix product with int32 accumulation and parity
- * extraction; it requires k <= 64 (else SCRS_ERR_CAPACITY{a = 64}) and
+ * extraction; it requires k <= 64 (else SCRS_ERR_CAPACITY{a = 64}).
+ * BITMATRIX_FP4 computes that product with 0/1 products on 4-bit operands
+ * and exact fp32 sums (k <= 256, so every GF(256) geometry).  Both require
  * symbol_len % 4 == 0 (else SCRS_ERR_INVALID_ARGUMENT{a = 2}).  Calls whose
  * buffers are not 4-byte aligned, scrs_decode_batch_ptrs_dev and the
  * streaming codecs use the table kernel.  Tower and AFFT handles:

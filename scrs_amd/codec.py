@@ -41,10 +41,11 @@ class Engine(IntEnum):
 class Kernel(IntEnum):
     """Multiply-XOR kernel of the GF(256) batch codecs (no reference
     counterpart): split-nibble LDS tables, or the GF(2) bit-matrix product on
-    int8 MFMA.  Both give the same bytes."""
+    int8 MFMA or on e2m1 (fp4) MFMA.  All give the same bytes."""
 
     TABLE = 0
     BITMATRIX = 1
+    BITMATRIX_FP4 = 2
 
 
 # ---------------------------------------------------------------------------

@@ -253,9 +253,9 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
     a.count = count;
     a.dst_aligned4 = aligned4(repairs_dev) && L % 4 == 0;
     hipError_t err;
-    if (enc->kernel == SCRS_KERNEL_BITMATRIX && aligned4(data_dev) && aligned4(repairs_dev)) {
+    if (enc->kernel != SCRS_KERNEL_TABLE && aligned4(data_dev) && aligned4(repairs_dev)) {
         a.tables = enc->d_bm_tables.as<uint32_t>();
-        err = launch_rows_bitmatrix(a, static_cast<hipStream_t>(stream));
+        err = launch_rows_bitmatrix(a, static_cast<hipStream_t>(stream), static_cast<uint32_t>(enc->kernel));
     } else {
         hipStream_t s = static_cast<hipStream_t>(stream);
         auto *mut = const_cast<scrs_batch_encoder *>(enc);
@@ -268,10 +268,12 @@ extern "C" scrs_status scrs_encode_batch_dev(const scrs_batch_encoder *enc, cons
 
 // Validation shared by both set_kernel calls.
 static scrs_status check_kernel(const scrs_profile &p, int32_t kernel) {
-    if (kernel != SCRS_KERNEL_TABLE && kernel != SCRS_KERNEL_BITMATRIX) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    if (kernel != SCRS_KERNEL_TABLE && kernel != SCRS_KERNEL_BITMATRIX && kernel != SCRS_KERNEL_BITMATRIX_FP4)
+        return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     if (!engine_is_gf256(p.engine)) return st(SCRS_CONFIG_UNSUPPORTED_MODE, p.engine);
-    if (kernel == SCRS_KERNEL_BITMATRIX) {
-        if (p.k > 8 * kBitmatrixMaxChunks) return st(SCRS_ERR_CAPACITY, 8 * kBitmatrixMaxChunks);
+    if (kernel != SCRS_KERNEL_TABLE) {
+        const uint32_t cap = bitmatrix_max_terms(static_cast<uint32_t>(kernel));
+        if (p.k > cap) return st(SCRS_ERR_CAPACITY, cap);
         if (p.symbol_len % 4) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     }
     return ok();
@@ -281,16 +283,17 @@ extern "C" scrs_status scrs_batch_encoder_set_kernel(scrs_batch_encoder *enc, in
     if (!enc) return st(SCRS_ERR_INVALID_ARGUMENT, 1);
     scrs_status s = check_kernel(enc->prof, kernel);
     if (s.code != SCRS_OK) return s;
-    if (kernel == SCRS_KERNEL_BITMATRIX && !enc->d_bm_tables.ptr) {
+    if (kernel != SCRS_KERNEL_TABLE && enc->bm_kind != kernel) {
         const uint32_t k = static_cast<uint32_t>(enc->prof.k), m = static_cast<uint32_t>(enc->prof.m);
         const std::vector<uint8_t> c = enc->prof.engine == SCRS_ENGINE_GOOD_CAUCHY ? good_cauchy_matrix(k, m)
                                                                                    : standard_cauchy_matrix(k, m);
-        std::vector<uint8_t> tab(bitmatrix_table_bytes(k, enc->groups));
-        build_bitmatrix_tables(c.data(), k, m, enc->groups, tab.data());
+        std::vector<uint8_t> tab(bitmatrix_table_bytes(k, enc->groups, static_cast<uint32_t>(kernel)));
+        build_bitmatrix_tables(c.data(), k, m, enc->groups, tab.data(), static_cast<uint32_t>(kernel));
         DeviceGuard g(enc->device);
         hipError_t err = enc->d_bm_tables.ensure(tab.size());
         if (err == hipSuccess) err = hipMemcpy(enc->d_bm_tables.ptr, tab.data(), tab.size(), hipMemcpyHostToDevice);
         if (err != hipSuccess) return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
+        enc->bm_kind = kernel;
     }
     enc->kernel = kernel;
     return ok();
@@ -432,7 +435,7 @@ static scrs_status decode_host(scrs_batch_decoder *dec, const uint64_t *indices,
                                                  nullptr, nullptr, sc.d_out.as<uint8_t>(), 1, s)
                               : dec->eng.enqueue(idx16.data(), sc.d_ptrs.as<const uint8_t *const>(), nullptr,
                                                  nullptr, sc.d_out.as<uint8_t>(), 1, s, L % 16 == 0,
-                                                 dec->kernel == SCRS_KERNEL_BITMATRIX);
+                                                 static_cast<uint32_t>(dec->kernel));
     if (r.code != SCRS_OK) return r;
     uint32_t flag = 0;
     if ((err = hipMemcpyAsync(sc.h_out.ptr, sc.d_out.ptr, k * L, hipMemcpyDeviceToHost, s)) != hipSuccess ||
@@ -482,8 +485,9 @@ static scrs_status decode_batch(scrs_batch_decoder *dec, const uint16_t *indices
     // only for the layout form, whose alignment is known here.
     const bool vec16 = !payload_ptrs_dev && L % 16 == 0 && aligned16(out_dev) && aligned16(data_dev) &&
                        aligned16(repairs_dev);
-    const bool bitmatrix = dec->kernel == SCRS_KERNEL_BITMATRIX && !payload_ptrs_dev && aligned4(out_dev) &&
-                           aligned4(data_dev) && aligned4(repairs_dev);
+    const uint32_t bitmatrix = (!payload_ptrs_dev && aligned4(out_dev) && aligned4(data_dev) && aligned4(repairs_dev))
+                                   ? static_cast<uint32_t>(dec->kernel)
+                                   : 0u;
     return dec->eng.enqueue(indices, payload_ptrs_dev, data_dev, repairs_dev, out_dev, count,
                             static_cast<hipStream_t>(stream), vec16, bitmatrix);
 }

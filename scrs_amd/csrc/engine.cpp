@@ -146,8 +146,9 @@ scrs_status DecodeEngine::init_device(const std::vector<uint16_t> &coords) {
     }
     // Bound the per-launch workspace: product tables dominate (the larger
     // of the two GF(2^8) kernels' layouts).
-    uint64_t table_bytes = cw_table_bytes(false);
-    if (!gf16 && k <= 8 * kBitmatrixMaxChunks && cw_table_bytes(true) > table_bytes) table_bytes = cw_table_bytes(true);
+    uint64_t table_bytes = cw_table_bytes(0);
+    for (uint32_t bm : {kBitmatrixInt8, kBitmatrixFp4})
+        if (!gf16 && k <= bitmatrix_max_terms(bm) && cw_table_bytes(bm) > table_bytes) table_bytes = cw_table_bytes(bm);
     const uint64_t budget = 1ull << 30;
     chunk = table_bytes ? budget / table_bytes : 65535;
     if (chunk > 65535) chunk = 65535;
@@ -156,9 +157,9 @@ scrs_status DecodeEngine::init_device(const std::vector<uint16_t> &coords) {
 
 scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs,
                                   const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out,
-                                  uint64_t count, hipStream_t s, bool vec16, bool bitmatrix) {
+                                  uint64_t count, hipStream_t s, bool vec16, uint32_t bitmatrix) {
     if (count == 0) return ok();
-    if (bitmatrix && (gf16 || k > 8 * kBitmatrixMaxChunks || L % 4)) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
+    if (bitmatrix && (gf16 || k > bitmatrix_max_terms(bitmatrix) || L % 4)) return st(SCRS_ERR_INVALID_ARGUMENT, 2);
     const uint64_t tb = cw_table_bytes(bitmatrix);
     if (chunk == 0) return st(SCRS_ERR_CAPACITY, tb);
     const uint64_t want = count < chunk ? count : chunk;
@@ -176,7 +177,7 @@ scrs_status DecodeEngine::enqueue(const uint16_t *indices, const uint8_t *const 
         cap = want;
     }
     // HBM product tables only for the kernels that read them.
-    const bool hbm_tables = bitmatrix || write_tables;
+    const bool hbm_tables = bitmatrix != 0 || write_tables;
     if (hbm_tables && (err = d_tables.ensure(cap * tb)) != hipSuccess)
         return st(err == hipErrorOutOfMemory ? SCRS_ERR_OUT_OF_MEMORY : SCRS_ERR_DEVICE, err);
     DecodeWorkspace ws;
@@ -241,7 +242,7 @@ DecodeWorkspace DecodeEngine::carve_workspace(uint8_t *base, uint64_t count) con
 
 scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *const *payload_ptrs,
                                     const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out, uint64_t cn,
-                                    hipStream_t s, bool vec16, bool bitmatrix, bool may_allocate) {
+                                    hipStream_t s, bool vec16, uint32_t bitmatrix, bool may_allocate) {
     const uint64_t tb = cw_table_bytes(bitmatrix);
     // Table path (either field): the multiply-XOR builds its product tables
     // in LDS from the fused coefficients, so none are written to HBM, unless
@@ -287,7 +288,7 @@ scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *co
     if (bitmatrix) {
         // Bit-matrix operands from the solve; the prepare step then
         // writes only the pointer tables.
-        if ((err = launch_bitmatrix_tables(pa, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+        if ((err = launch_bitmatrix_tables(pa, s, bitmatrix)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
         pa.fused = nullptr;
     } else if (fused_in_rows) {
         pa.fused = nullptr;
@@ -310,7 +311,7 @@ scrs_status DecodeEngine::run_chunk(const DecodeWorkspace &ws, const uint8_t *co
     // Missing rows are out + index * L.
     ra.dst_aligned4 = (reinterpret_cast<uintptr_t>(out) & 3u) == 0 && L % 4 == 0;
     const bool attached = !bitmatrix && split.acquire(ra, force_split && rows_hint ? rows_hint : rmax, s, may_allocate);
-    err = bitmatrix ? launch_rows_bitmatrix(ra, s) : launch_rows_terms_field(gf16, ra, vec16, s);
+    err = bitmatrix ? launch_rows_bitmatrix(ra, s, bitmatrix) : launch_rows_terms_field(gf16, ra, vec16, s);
     if (attached) split.release(s);
     return err == hipSuccess ? ok() : st(SCRS_ERR_DEVICE, err);
 }

@@ -120,11 +120,12 @@ struct DecodeEngine {
     // count x k, received order) are already validated.  Received payloads
     // come from `payload_ptrs` (device array, count x k) or from the
     // back-to-back data / repair layout.  Output rows: out + c*k*L + i*L.
-    // `bitmatrix` runs the GF(2^8) multiply-XOR on the bit-matrix kernel
-    // (payload rows and outputs 4-byte aligned, k <= 64).
+    // `bitmatrix` (kBitmatrixInt8 or kBitmatrixFp4, 0 for the tables) runs
+    // the GF(2^8) multiply-XOR on that bit-matrix kernel (payload rows and
+    // outputs 4-byte aligned, k <= bitmatrix_max_terms).
     scrs_status enqueue(const uint16_t *indices, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
                         const uint8_t *repair_base, uint8_t *out, uint64_t count, hipStream_t s, bool vec16,
-                        bool bitmatrix = false);
+                        uint32_t bitmatrix = 0);
     // The chain of one launch chunk of `cn` codewords whose indices are
     // already in ws.idx (payload, data, repair and output bases of the
     // chunk's first codeword).
@@ -132,7 +133,7 @@ struct DecodeEngine {
     // if it already holds the launch, else atomics.
     scrs_status run_chunk(const DecodeWorkspace &ws, const uint8_t *const *payload_ptrs, const uint8_t *data_base,
                           const uint8_t *repair_base, uint8_t *out, uint64_t cn, hipStream_t s, bool vec16,
-                          bool bitmatrix, bool may_allocate = true);
+                          uint32_t bitmatrix, bool may_allocate = true);
     // Bytes a caller-provided workspace needs for `count` codewords of the
     // GF(2^8) table kernel (no product tables in HBM): each array starts on
     // a 256-byte boundary.  carve_workspace lays them out from `base`.
@@ -144,9 +145,10 @@ struct DecodeEngine {
     scrs_status enqueue_recipe(const uint16_t *indices, const uint32_t *tables, const uint16_t *missing,
                                const uint32_t *rows, const uint8_t *data_base, const uint8_t *repair_base,
                                uint8_t *out, hipStream_t s, bool vec16);
-    // Table bytes per codeword of the table or bit-matrix kernel.
-    uint64_t cw_table_bytes(bool bitmatrix) const {
-        return bitmatrix ? bitmatrix_table_bytes(k, groups) : static_cast<uint64_t>(k) * groups * table_group_bytes(gf16);
+    // Table bytes per codeword of the table kernel (0) or a bit-matrix kernel.
+    uint64_t cw_table_bytes(uint32_t bitmatrix) const {
+        return bitmatrix ? bitmatrix_table_bytes(k, groups, bitmatrix)
+                         : static_cast<uint64_t>(k) * groups * table_group_bytes(gf16);
     }
     // Bytes of one codeword's recipe: tables, missing rows, row count.
     uint64_t recipe_table_bytes() const { return static_cast<uint64_t>(k) * groups * table_group_bytes(gf16); }

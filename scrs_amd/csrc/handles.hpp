@@ -26,7 +26,8 @@ struct scrs_batch_encoder {
     uint32_t groups = 0;
     scrs::DevBuf d_tables;       // GF(2^8) product tables of all k terms
     int32_t kernel = SCRS_KERNEL_TABLE;
-    scrs::DevBuf d_bm_tables;    // bit-matrix operands (SCRS_KERNEL_BITMATRIX)
+    scrs::DevBuf d_bm_tables;    // bit-matrix operands of the kernel bm_kind
+    int32_t bm_kind = SCRS_KERNEL_TABLE;  // none built yet
     scrs::SplitBuffer split;     // partial rows of launches too small to fill the device
     hipStream_t stream = nullptr;  // host-slice calls only, created on first use
     std::once_flag stream_once;

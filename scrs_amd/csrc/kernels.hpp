@@ -141,25 +141,31 @@ struct DecodePrepArgs {
     uint64_t L = 0, count = 0;
 };
 hipError_t launch_decode_prepare(const DecodePrepArgs &a, bool gf16, hipStream_t s);
-// Bit-matrix decode tables from a GF(2^8) solve (tables only; the pointer
-// tables come from launch_decode_prepare with fused = nullptr).
-hipError_t launch_bitmatrix_tables(const DecodePrepArgs &a, hipStream_t s);
+// Bit-matrix decode tables (kind: kBitmatrixInt8 or kBitmatrixFp4) from a
+// GF(2^8) solve (tables only; the pointer tables come from
+// launch_decode_prepare with fused = nullptr).
+hipError_t launch_bitmatrix_tables(const DecodePrepArgs &a, hipStream_t s, uint32_t kind);
 
 // GF(2^16) multiply-XOR over interleaved little-endian u16 elements, same
 // addressing as RowsTermsArgs; tables in the 2048 B per (term, group) layout
 // (build_nibble_tables16).
 hipError_t launch_rows_terms16(const RowsTermsArgs &a, bool vec16, hipStream_t s);
 
-// Bit-matrix path: the same multiply-XOR as a GF(2) product on the int8
-// matrix cores.  Tables hold, per codeword, row group g and 8-term chunk ch,
-// the MFMA matrix operands of the 8 output bit planes (8 KiB per chunk):
-// bitmatrix_table_bytes(T, groups) bytes per codeword.  Requires T <= 64,
-// L % 4 == 0 and 4-byte aligned rows; no accumulate, no term_index.
+// Bit-matrix path: the same multiply-XOR as a GF(2) product on the matrix
+// cores, with int8 operands (kBitmatrixInt8, 8-term chunks, T <= 64) or
+// e2m1 operands (kBitmatrixFp4, 16-term chunks, T <= 256).  Tables hold,
+// per codeword, row group g and chunk ch, the MFMA matrix operands of the 8
+// output bit planes (8 KiB per chunk): bitmatrix_table_bytes(T, groups, kind)
+// bytes per codeword.  Requires L % 4 == 0 and 4-byte aligned rows; no
+// accumulate, no term_index.  The kinds equal the C ABI's kernel selectors.
+constexpr uint32_t kBitmatrixInt8 = 1, kBitmatrixFp4 = 2;
 constexpr uint32_t kBitmatrixMaxChunks = 8;
-uint64_t bitmatrix_table_bytes(uint32_t T, uint32_t groups);
+constexpr uint32_t kFp4MaxChunks = 16;
+uint32_t bitmatrix_max_terms(uint32_t kind);
+uint64_t bitmatrix_table_bytes(uint32_t T, uint32_t groups, uint32_t kind);
 // Host build for a T x R block given term-major (coef[t*R + r]).
-void build_bitmatrix_tables(const uint8_t *coef, uint32_t T, uint32_t R, uint32_t groups, uint8_t *out);
-hipError_t launch_rows_bitmatrix(const RowsTermsArgs &a, hipStream_t s);
+void build_bitmatrix_tables(const uint8_t *coef, uint32_t T, uint32_t R, uint32_t groups, uint8_t *out, uint32_t kind);
+hipError_t launch_rows_bitmatrix(const RowsTermsArgs &a, hipStream_t s, uint32_t kind);
 
 // Fill `len` bytes with the seeded splitmix64 counter stream: byte i is byte
 // (i % 8) of splitmix64 word i / 8 drawn from state seed + (i/8) * golden.

@@ -121,26 +121,61 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
+// Byte B of x, zero-extended, AND m: one SDWA instruction (src0_sel BYTE_B).
+// Written out because the compiler folds only some shift-and-mask pairs into
+// SDWA operands.  A VALU instruction, no memory access.
+template <int B>
+__device__ __forceinline__ uint32_t byte_and(uint32_t x, uint32_t m) {
+    uint32_t r;
+    if constexpr (B == 0) r = x & m;
+    else if constexpr (B == 1)
+        asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 src1_sel:DWORD"
+            : "=v"(r) : "v"(x), "s"(m));
+    else if constexpr (B == 2)
+        asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2 src1_sel:DWORD"
+            : "=v"(r) : "v"(x), "s"(m));
+    else
+        asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_3 src1_sel:DWORD"
+            : "=v"(r) : "v"(x), "s"(m));
+    return r;
+}
+
+// One source byte (byte B of the dword whose rotated copies are lo and hi)
+// folded into its 16 row accumulators.
+template <int RW, int B>
+__device__ __forceinline__ void fold_byte(uint32_t (&acc)[RW / 4], const uint8_t *tab, uint32_t lo, uint32_t hi) {
+    constexpr int SH = RW == 16 ? 4 : (RW == 8 ? 3 : 2);  // log2(RW)
+    constexpr uint32_t BMASK = 0x0Fu << SH;                 // one entry offset
+    Entry<RW> el, eh;
+    if constexpr (SH == 4) {
+        el.load(tab, byte_and<B>(lo, BMASK));
+        eh.load(tab + 16 * RW, byte_and<B>(hi, BMASK));
+    } else {
+        el.load(tab, (lo >> (8 * B)) & BMASK);
+        eh.load(tab + 16 * RW, (hi >> (8 * B)) & BMASK);
+    }
+#pragma unroll
+    for (int d = 0; d < RW / 4; d++) acc[d] = xor3(acc[d], el.d[d], eh.d[d]);
+}
+
 // Fold one 16-byte source slice into the accumulators.  `tab` points at the
 // term's 32 entries (lo 0..15, hi 16..31) of RW bytes each.
 template <int RW>
 __device__ __forceinline__ void fold_term(uint32_t (&acc)[16][RW / 4], const uint8_t *tab, const uint32_t (&w)[4]) {
     constexpr int SH = RW == 16 ? 4 : (RW == 8 ? 3 : 2);  // log2(RW)
-    constexpr uint32_t BMASK = 0x0Fu << SH;                 // one entry offset
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         // Rotations place each byte's low (high) nibble times RW into bits
-        // SH..SH+3 of that byte; masking byte b gives the entry offset.
-        const uint32_t lo = __builtin_amdgcn_alignbit(w[j], w[j], 32 - SH);
+        // SH..SH+3 of that byte; masking byte b gives the entry offset.  With
+        // 16-byte entries (SH = 4) the high nibble is in place already and a
+        // left shift by 4 places the low one, and each offset is one SDWA
+        // AND of byte b (byte_and).
+        const uint32_t lo = SH == 4 ? w[j] << 4 : __builtin_amdgcn_alignbit(w[j], w[j], 32 - SH);
         const uint32_t hi = SH == 4 ? w[j] : __builtin_amdgcn_alignbit(w[j], w[j], 4 - SH);
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            Entry<RW> el, eh;
-            el.load(tab, (lo >> (8 * b)) & BMASK);
-            eh.load(tab + 16 * RW, (hi >> (8 * b)) & BMASK);
-#pragma unroll
-            for (int d = 0; d < RW / 4; d++) acc[4 * j + b][d] = xor3(acc[4 * j + b][d], el.d[d], eh.d[d]);
-        }
+        fold_byte<RW, 0>(acc[4 * j + 0], tab, lo, hi);
+        fold_byte<RW, 1>(acc[4 * j + 1], tab, lo, hi);
+        fold_byte<RW, 2>(acc[4 * j + 2], tab, lo, hi);
+        fold_byte<RW, 3>(acc[4 * j + 3], tab, lo, hi);
         // Fold four columns at a time: keeps the lookups in flight bounded
         // and the register budget at four waves per SIMD.
         __builtin_amdgcn_sched_barrier(0);
