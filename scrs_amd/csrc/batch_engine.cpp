@@ -616,13 +616,14 @@ hipError_t BatchEngine::launch(Slot &slot, const Plan &p, uint32_t tile, hipStre
     const bool reduce = !p.red.empty();
     if (cfg_.use_graph) {
         err = run_graph(slot, tile, reduce, s);
+        if (err != hipSuccess) return err;
+        slot.claims += p.units;
     } else {
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(batch_rows_grid(tile), p.units));
-        err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, grid, s);
-        if (err == hipSuccess && reduce) err = launch_batch_reduce(dh, batch_reduce_grid(p.red_items), s);
+        if ((err = launch_batch_rows(dh, slot.counter.as<uint32_t>(), tile, grid, s)) != hipSuccess) return err;
+        slot.claims += p.units;  // the launch is enqueued: it will draw nunits times
+        if (reduce && (err = launch_batch_reduce(dh, batch_reduce_grid(p.red_items), s)) != hipSuccess) return err;
     }
-    if (err != hipSuccess) return err;
-    slot.claims += p.units;
     slot.seq = seq_;
     return hipEventRecord(slot.done, s);
 }

@@ -237,7 +237,7 @@ def test_server_admission_budget(torch_cuda):
 @pytest.mark.parametrize("engine", ENGINES)
 def test_lone_large_jobs_split_terms(torch_cuda, engine):
     # A batch far too small to fill the device splits each job's term loop
-    # across units whose partial rows are XORed into cleared destinations.
+    # across units that store partial rows; a reduce launch combines them.
     torch = torch_cuda
     rng = np.random.default_rng(int(engine) + 40)
     m = 32 if engine != Engine.GOOD_CAUCHY else 30
@@ -260,3 +260,21 @@ def test_lone_large_jobs_split_terms(torch_cuda, engine):
         eng.submit([Job.decode(p, idx, ptrs, out)])
         torch.cuda.synchronize()
         assert torch.equal(out, data)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_unit_counter_across_many_batches(torch_cuda, use_graph):
+    """The multiply-XOR draws its units from a counter that only grows: each
+    launch starts from the value the host recorded after the previous one.
+    Thirty batches of 1, 5 and 40 jobs (split and unsplit ranges) through one
+    engine, every output checked after each batch."""
+    torch = torch_cuda
+    loads = [Workload(torch, 50 + i, n, lens=(512, 1500, 4096)) for i, n in enumerate((1, 5, 40))]
+    eng = BatchEngine(tile_bytes=4096, cw_per_unit=1, use_graph=use_graph)
+    for b in range(30):
+        w = loads[b % 3]
+        w.clear()
+        eng.submit(w.jobs)
+        torch.cuda.synchronize()
+        w.check()
+    assert eng.stats()["batches"] == 30
