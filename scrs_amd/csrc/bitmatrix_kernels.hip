@@ -447,7 +447,7 @@ __global__ __launch_bounds__(256) void bitmatrix_fp4_tables_kernel(DecodePrepArg
 
 template <int NCH>
 hipError_t launch_fp4_nch(const RowsTermsArgs &a, bool dec, dim3 grid, uint32_t tiles, uint32_t units, hipStream_t s) {
-    constexpr int W = 1;
+    constexpr int W = 1;  // 8-byte source loads (W = 2) spill at these register budgets
     constexpr int RING = NCH <= 4 ? SCRS_FP4_RING : 2;  // registers allow the deep ring up to 4 chunks
     const size_t shmem = NCH * kBmChunkV4 * sizeof(v4i);
     hipError_t err =
