@@ -1,5 +1,5 @@
 // bitmatrix_kernels.hip - GF(2^8) multiply-XOR as a GF(2) bit-matrix product
-// on the int8 matrix cores of gfx950.
+// on the matrix cores of gfx950, with int8 or e2m1 (fp4) operands.
 //
 // Over GF(2) a GF(2^8) code of T sources and R rows is an (8R) x (8T) 0/1
 // matrix M acting on the bit vector of every byte column:
