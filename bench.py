@@ -48,17 +48,25 @@ def afft_table_reads(k: int, m: int, sl: int) -> tuple[int, int]:
     over the coset, in passes of (8, Dk-8) levels.  Decode over N = k + m
     rows: inverse in two 8-level passes (input scale, then the T(q) scale),
     the XOR-only derivative pass, the forward over the high bits (1/T scale)
-    and the forward over the low bits of the rows below k (output scale)."""
+    and the forward over the low bits of the rows below k (output scale).
+    The first node of every level has factor 0: in the top stage of the
+    top pass (its upper 4 levels, 3 in a 7-level pass) of a transform that
+    starts at point 0, the 2^B butterflies of level B on that node skip
+    their product.  The encode's inverse and the decode's inverse and
+    forward start at 0; the encode's forward starts at k."""
     assert k == m and k & (k - 1) == 0 and 2 * k == 1 << 16
     el = sl // 2
     per = lambda g: 3 if g == 8 else 4  # noqa: E731
     dk = k.bit_length() - 1
+    skipped = lambda top, levels: sum(1 << b for b in range(top - levels, top))  # noqa: E731
     enc = (8 * per(8) + 2 * (dk - 8) * per(dk - 8) + 8 * per(8)) * (k // 2) * el
+    enc -= skipped(dk, 3) * per(dk - 8) * el
     n = 2 * k
     inv = 2 * 8 * per(8) * (n // 2) * el + 2 * n * el * 4
     fwd_hi = 8 * per(8) * (n // 2) * el + n * el * 4
     fwd_lo = 8 * per(8) * (k // 2) * el + k * el * 4
-    return enc, inv + fwd_hi + fwd_lo
+    dec = inv + fwd_hi + fwd_lo - 2 * skipped(16, 4) * per(8) * el
+    return enc, dec
 
 
 def erasure_indices(count: int, k: int, m: int, max_r: int, seed: int) -> np.ndarray:

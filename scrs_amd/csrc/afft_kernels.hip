@@ -362,9 +362,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         asm("" : "+s"(lb));
         return lb;
     };
+    // The first node of every level has the factor W(0) = 0.  In the pass's
+    // top stage no pass bit lies above the stage, so a butterfly sits on its
+    // level's first node exactly when the point base of the group has no bit
+    // above the level and the register index none above the butterfly bit.
+    // When the first holds for every level of the stage (uniform over the
+    // workgroup), the stage runs in a form whose butterflies on the first
+    // node skip their product (the reads and the combine) at compile time;
+    // the XOR of the pair remains.
+    auto zero_stage = [&](uint32_t pbase, uint32_t s_lo, uint32_t t) {
+        return s_lo + t == g && (pbase >> (b_lo + s_lo + 1u)) == 0u;
+    };
     // The butterflies of a level are software pipelined: the table reads of
     // product n are issued before product n - kPipe is combined.
-    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic) {
+    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc) {
         constexpr int T = decltype(tc)::value;
         constexpr bool inverse = decltype(ic)::value;
         constexpr int NU = R >> T;
@@ -377,18 +388,19 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             // Butterfly n of the level: registers jn and jn + 2^bs.
             auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
             auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kTN * 2); };
+            auto zero = [&](int j) { return decltype(zc)::value && off(j) == 0; };
             Parts<W> pp[kPipe + 1];
 #pragma unroll
             for (int n = 0; n < R / 2 + kPipe; n++) {
                 if (n < R / 2) {
                     const int j = jn(n);
                     if constexpr (inverse) v[j + (1 << bs)] ^= v[j];
-                    gmul2_issue<W>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
+                    if (!zero(j)) gmul2_issue<W>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 if (n >= kPipe) {
                     const int j = jn(n - kPipe);
-                    v[j] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
+                    if (!zero(j)) v[j] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
                     if constexpr (!inverse) v[j + (1 << bs)] ^= v[j];
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -396,21 +408,30 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         }
     };
     // Full stages, and 3-bit stages (a 7-level pass), take the fast path.
-    auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, auto ic) {
+    // The 8-level narrow-table form keeps one form of its stages: with both,
+    // it needs more than the 128 registers of 4 waves per SIMD.
+    auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, bool zs, auto ic) {
+        auto run = [&](auto tc) {
+            if (zs && (W || GL == 0))
+                stage_fast(tabs, s_lo, tc, ic, std::true_type{});
+            else
+                stage_fast(tabs, s_lo, tc, ic, std::false_type{});
+        };
         if (t == RB) {
-            stage_fast(tabs, s_lo, std::integral_constant<int, static_cast<int>(RB)>{}, ic);
+            run(std::integral_constant<int, static_cast<int>(RB)>{});
             return true;
         }
         if constexpr (GL == 0 && RB == 4) {
             if (t == 3) {
-                stage_fast(tabs, s_lo, std::integral_constant<int, 3>{}, ic);
+                run(std::integral_constant<int, 3>{});
                 return true;
             }
         }
         return false;
     };
+    const uint32_t inv_pbase = a.inv_base + base, fwd_pbase = a.fwd_base + base;
     auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
-        if (fast_stage(inv_t, s_lo, t, std::true_type{})) return;
+        if (fast_stage(inv_t, s_lo, t, zero_stage(inv_pbase, s_lo, t), std::true_type{})) return;
 #pragma unroll
         for (int bs = 0; bs < static_cast<int>(RB); bs++) {
             if (static_cast<uint32_t>(bs) >= t) break;
@@ -424,7 +445,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         }
     };
     auto fwd_stage = [&](uint32_t s_lo, uint32_t t) {
-        if (fast_stage(fwd_t, s_lo, t, std::false_type{})) return;
+        if (fast_stage(fwd_t, s_lo, t, zero_stage(fwd_pbase, s_lo, t), std::false_type{})) return;
 #pragma unroll
         for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
             if (static_cast<uint32_t>(bs) >= t) continue;
