@@ -29,6 +29,21 @@ namespace {
 constexpr uint32_t kTileDwords = 64;  // 256-byte column tile
 constexpr uint32_t kTab = 64;         // u16 entries per constant table
 constexpr uint32_t kTabW = 128;       // u16 entries per wide constant table
+// Pass features, for kernels specialised on them.
+enum : uint32_t {
+    kFxInv = 1u << 0,      // inverse levels
+    kFxFwd = 1u << 1,      // forward levels
+    kFxDer = 1u << 2,      // derivative with its factors
+    kFxUnit = 1u << 3,     // derivative with unit factors
+    kFxSin = 1u << 4,      // input row scale
+    kFxSout = 1u << 5,     // output row scale
+    kFxSder = 1u << 6,     // derivative row scale
+    kFxCopy = 1u << 7,     // copies of the loaded rows
+    kFxAux = 1u << 8,      // derivative accumulator rows
+    kFxInRows = 1u << 9,   // input row table
+    kFxOutRows = 1u << 10, // output row table
+    kFxDynamic = ~0u,      // features read from the arguments
+};
 
 typedef __attribute__((address_space(4))) const uint64_t *const_u64_ptr;
 template <typename T>
@@ -224,9 +239,32 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 // A 16-row pass stages at least 64 KiB of LDS, so a CU holds one 1024-thread
 // workgroup: 4 waves per SIMD.  Telling the compiler so lets it spend the
 // registers of that occupancy on table reads in flight.
-template <int R, bool A4, uint32_t GL, bool W>
+//
+// FX: the pass's features fixed at compile time (kFx* bits), or kFxDynamic
+// to read them from the arguments.  A fixed set drops the branches of the
+// absent features from the tile loop, and with them the register and
+// wait-count bookkeeping the compiler keeps for every path: the flags and
+// the absent pointers become constants, the present pointers non-null.
+template <int R, bool A4, uint32_t GL, bool W, uint32_t FX>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_WAVES_PER_EU, SCRS_AFFT_WAVES_PER_EU))) void afft_pass_kernel(
     AfftPassArgs a, uint32_t tpb) {
+    if constexpr (FX != kFxDynamic) {
+        a.flags = ((FX & kFxInv) ? kAfftInv : 0u) | ((FX & kFxFwd) ? kAfftFwd : 0u) |
+                  ((FX & kFxDer) ? kAfftDeriv : 0u) | ((FX & kFxUnit) ? kAfftDerivUnit : 0u);
+        auto fix = [](auto &ptr, bool present) {
+            if (present)
+                __builtin_assume(ptr != nullptr);
+            else
+                ptr = nullptr;
+        };
+        fix(a.scale_in, FX & kFxSin);
+        fix(a.scale_out, FX & kFxSout);
+        fix(a.scale_der, FX & kFxSder);
+        fix(a.copy_rows, FX & kFxCopy);
+        fix(a.aux_base, FX & kFxAux);
+        fix(a.in_rows, FX & kFxInRows);
+        fix(a.out_rows, FX & kFxOutRows);
+    }
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
     constexpr uint32_t kTN = W ? kTabW : kTab;          // u16 entries per node table
     extern __shared__ __align__(256) uint32_t smem[];  // tables at multiples of 128 (wide: 256) bytes
@@ -583,7 +621,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     }
 }
 
-template <int R, bool A4, uint32_t GL = 0, bool W = false>
+template <int R, bool A4, uint32_t GL = 0, bool W = false, uint32_t FX = kFxDynamic>
 hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
     const uint32_t G = 1u << a.g;
     const uint64_t tiles = (a.L + kTileDwords * 4 - 1) / (kTileDwords * 4);
@@ -601,9 +639,9 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W>), 160 * 1024);
+    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W, FX>), 160 * 1024);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL, W>), grid, block, lds, s, a, tpb);
+    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL, W, FX>), grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
 
@@ -759,6 +797,15 @@ size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale
     return b;
 }
 
+// The kFx* feature set of a pass's arguments.
+uint32_t pass_features(const AfftPassArgs &a) {
+    return ((a.flags & kAfftInv) ? kFxInv : 0u) | ((a.flags & kAfftFwd) ? kFxFwd : 0u) |
+           ((a.flags & kAfftDeriv) ? kFxDer : 0u) | ((a.flags & kAfftDerivUnit) ? kFxUnit : 0u) |
+           (a.scale_in ? kFxSin : 0u) | (a.scale_out ? kFxSout : 0u) | (a.scale_der ? kFxSder : 0u) |
+           (a.copy_rows ? kFxCopy : 0u) | (a.aux_base ? kFxAux : 0u) | (a.in_rows ? kFxInRows : 0u) |
+           (a.out_rows ? kFxOutRows : 0u);
+}
+
 hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, hipStream_t s) {
     if (a.g == 0 || a.g > 8 || a.b_lo + a.g > D || D > kAfftMaxLog || count == 0 || count > 65535 || a.L == 0)
         return hipErrorInvalidValue;
@@ -780,8 +827,24 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
         const size_t wgs = (160u * 1024u) / bytes, waves = (size_t{1} << a.g) / 16;
         return std::min<size_t>(16, wgs * waves);
     };
+    const uint32_t fx = pass_features(a);
     if (a4 && a.g >= 4 && a.node_tabs_w && lds_w <= 160 * 1024 && waves_per_cu(lds_w) >= waves_per_cu(lds)) {
-        if (a.g == 8) return launch_pass_r<16, true, 8, true>(a, D, count, lds_w, s);
+        if (a.g == 8) {
+            // The feature sets of the encode and decode pass chains, except
+            // the decode's first pass (input row table, copies and input
+            // scale): fixed, its prefetched rows spill.
+            switch (fx) {
+                case kFxInv: return launch_pass_r<16, true, 8, true, kFxInv>(a, D, count, lds_w, s);
+                case kFxFwd: return launch_pass_r<16, true, 8, true, kFxFwd>(a, D, count, lds_w, s);
+                case kFxInv | kFxSout: return launch_pass_r<16, true, 8, true, kFxInv | kFxSout>(a, D, count, lds_w, s);
+                case kFxUnit: return launch_pass_r<16, true, 8, true, kFxUnit>(a, D, count, lds_w, s);
+                case kFxUnit | kFxFwd | kFxSder | kFxAux:
+                    return launch_pass_r<16, true, 8, true, kFxUnit | kFxFwd | kFxSder | kFxAux>(a, D, count, lds_w, s);
+                case kFxFwd | kFxOutRows | kFxSout:
+                    return launch_pass_r<16, true, 8, true, kFxFwd | kFxOutRows | kFxSout>(a, D, count, lds_w, s);
+                default: return launch_pass_r<16, true, 8, true>(a, D, count, lds_w, s);
+            }
+        }
         return launch_pass_r<16, true, 0, true>(a, D, count, lds_w, s);
     }
     if (a4) {
@@ -790,7 +853,9 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
             case 1: return launch_pass_r<2, true>(a, D, count, lds, s);
             case 2: return launch_pass_r<4, true>(a, D, count, lds, s);
             case 3: return launch_pass_r<8, true>(a, D, count, lds, s);
-            default: return launch_pass_r<16, true>(a, D, count, lds, s);
+            default:
+                if (fx == (kFxInv | kFxFwd)) return launch_pass_r<16, true, 0, false, kFxInv | kFxFwd>(a, D, count, lds, s);
+                return launch_pass_r<16, true>(a, D, count, lds, s);
         }
     }
     switch (a.g) {
