@@ -42,13 +42,14 @@ def afft_table_reads(k: int, m: int, sl: int) -> tuple[int, int]:
     """Table reads (one 2-byte LDS read per lane) of the additive-FFT encode
     and erasure decode at k = m = 2^D/2: a butterfly multiplies each 16-bit
     element of its high row by the node constant, 3 reads with the 6+5+5-bit
-    tables of 8-level passes, 4 with the nibble tables of 7-level passes
-    (whose smaller footprint doubles the waves per CU); a per-row scale is 4
-    reads per element.  Encode: inverse over the 2^Dk data rows and forward
-    over the coset, in passes of (8, Dk-8) levels.  Decode over N = k + m
-    rows: inverse in two 8-level passes (input scale, then the T(q) scale),
-    the XOR-only derivative pass, the forward over the high bits (1/T scale)
-    and the forward over the low bits of the rows below k (output scale).
+    tables; the 7-level encode pass (inverse and forward tables) keeps nibble
+    tables, 4 reads, on its first level so that it fits two workgroups per
+    CU; a per-row scale is 4 reads per element.  Encode: inverse over the
+    2^Dk data rows and forward over the coset, in passes of (8, Dk-8)
+    levels.  Decode over N = k + m rows: inverse in two 8-level passes
+    (input scale, then the T(q) scale), the XOR-only derivative pass, the
+    forward over the high bits (1/T scale) and the forward over the low bits
+    of the rows below k (output scale).
     The first node of every level has factor 0: in the top stage of the
     top pass (its upper 4 levels, 3 in a 7-level pass) of a transform that
     starts at point 0, the 2^B butterflies of level B on that node skip
@@ -56,16 +57,17 @@ def afft_table_reads(k: int, m: int, sl: int) -> tuple[int, int]:
     forward start at 0; the encode's forward starts at k."""
     assert k == m and k & (k - 1) == 0 and 2 * k == 1 << 16
     el = sl // 2
-    per = lambda g: 3 if g == 8 else 4  # noqa: E731
+    per = 3
     dk = k.bit_length() - 1
     skipped = lambda top, levels: sum(1 << b for b in range(top - levels, top))  # noqa: E731
-    enc = (8 * per(8) + 2 * (dk - 8) * per(dk - 8) + 8 * per(8)) * (k // 2) * el
-    enc -= skipped(dk, 3) * per(dk - 8) * el
+    mid = 4 + (dk - 9) * per  # the 7-level pass: nibble first level, wide above
+    enc = (8 * per + 2 * mid + 8 * per) * (k // 2) * el
+    enc -= skipped(dk, 3) * per * el
     n = 2 * k
-    inv = 2 * 8 * per(8) * (n // 2) * el + 2 * n * el * 4
-    fwd_hi = 8 * per(8) * (n // 2) * el + n * el * 4
-    fwd_lo = 8 * per(8) * (k // 2) * el + k * el * 4
-    dec = inv + fwd_hi + fwd_lo - 2 * skipped(16, 4) * per(8) * el
+    inv = 2 * 8 * per * (n // 2) * el + 2 * n * el * 4
+    fwd_hi = 8 * per * (n // 2) * el + n * el * 4
+    fwd_lo = 8 * per * (k // 2) * el + k * el * 4
+    dec = inv + fwd_hi + fwd_lo - 2 * skipped(16, 4) * per * el
     return enc, dec
 
 

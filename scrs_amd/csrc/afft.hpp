@@ -70,7 +70,7 @@ struct AfftPassArgs {
 // codewords.
 hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, hipStream_t s);
 size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der,
-                           bool wide = false);
+                           bool wide = false, bool mixed = false);
 
 // Nibble product tables: out[i*64 + q*16 + n] = c[i] * (n << 4q).
 hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s);

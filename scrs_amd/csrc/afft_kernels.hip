@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "afft.hpp"
 #include "device.hpp"
@@ -126,6 +127,17 @@ __device__ __forceinline__ uint32_t gmul2_at(uint32_t base, uint32_t off, uint32
 #define SCRS_AFFT_PIPE 1  // products in flight ahead of the one being combined
 #endif
 constexpr int kPipe = SCRS_AFFT_PIPE;
+
+// f(integral_constant<int, I>) for I = 0 .. N-1, so that I is a constant
+// expression inside f.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <bool W>
 struct Parts {
@@ -240,12 +252,17 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 // workgroup: 4 waves per SIMD.  Telling the compiler so lets it spend the
 // registers of that occupancy on table reads in flight.
 //
+// MX (with W): mixed node tables, nibble tables for the pass's first level
+// (half of the pass's nodes) and wide tables above it: three quarters of the
+// wide footprint, so that a 7-level pass with inverse and forward tables
+// still fits two workgroups per CU, with three reads per element on all but
+// its first level.
 // FX: the pass's features fixed at compile time (kFx* bits), or kFxDynamic
 // to read them from the arguments.  A fixed set drops the branches of the
 // absent features from the tile loop, and with them the register and
 // wait-count bookkeeping the compiler keeps for every path: the flags and
 // the absent pointers become constants, the present pointers non-null.
-template <int R, bool A4, uint32_t GL, bool W, uint32_t FX>
+template <int R, bool A4, uint32_t GL, bool W, bool MX, uint32_t FX>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_WAVES_PER_EU, SCRS_AFFT_WAVES_PER_EU))) void afft_pass_kernel(
     AfftPassArgs a, uint32_t tpb) {
     if constexpr (FX != kFxDynamic) {
@@ -267,6 +284,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     }
     constexpr uint32_t RB = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : (R >= 2 ? 1 : 0)));
     constexpr uint32_t kTN = W ? kTabW : kTab;          // u16 entries per node table
+    static_assert(!MX || W, "mixed tables extend the wide format");
     extern __shared__ __align__(256) uint32_t smem[];  // tables at multiples of 128 (wide: 256) bytes
     const uint32_t g = GL ? GL : a.g, G = 1u << g, b_lo = a.b_lo;
     const uint32_t gidx = blockIdx.y;
@@ -275,9 +293,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     const bool inv = a.flags & kAfftInv, fwd = a.flags & kAfftFwd;
     const bool unit = a.flags & kAfftDerivUnit, der = unit || (a.flags & kAfftDeriv);
     uint32_t *data = smem;
+    // Node tables: level beta's G >> (beta + 1) tables of tab_n(beta) u16
+    // entries from level_off(beta) on.
+    auto tab_n = [&](uint32_t beta) -> uint32_t { return (MX && beta == 0) ? kTab : kTN; };
+    auto level_off = [&](uint32_t beta) -> uint32_t {
+        const uint32_t below = G - (G >> beta);  // tables of the levels below beta
+        if constexpr (MX)
+            return beta == 0 ? 0u : (G / 2) * kTab + (below - G / 2) * kTabW;
+        else
+            return below * kTN;
+    };
+    const uint32_t tabs_n = level_off(g);
     uint16_t *inv_t = reinterpret_cast<uint16_t *>(smem + G * kTileDwords);
-    uint16_t *fwd_t = inv_t + (inv ? (G - 1) * kTN : 0);
-    uint16_t *der_t = fwd_t + (fwd ? (G - 1) * kTN : 0);
+    uint16_t *fwd_t = inv_t + (inv ? tabs_n : 0);
+    uint16_t *der_t = fwd_t + (fwd ? tabs_n : 0);
     uint16_t *sin_t = der_t + (der && !unit ? 16 * kTab : 0);
     uint16_t *sout_t = sin_t + (a.scale_in ? G * kTab : 0);
     uint16_t *sder_t = sout_t + (a.scale_out ? G * kTab : 0);
@@ -287,13 +316,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // uses G >> (beta+1) consecutive nodes.
     auto stage_nodes = [&](uint16_t *dst, uint32_t pbase) {
         uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(W ? a.node_tabs_w : a.node_tabs);
         for (uint32_t beta = 0; beta < g; beta++) {
             const uint32_t B = b_lo + beta;
             const uint32_t cnt = G >> (beta + 1);
             const uint32_t first = (1u << (15u - B)) + (pbase >> (B + 1));
-            const uint32_t off = (G - (G >> beta)) * (kTN / 2);
-            for (uint32_t i = tid; i < cnt * (kTN / 2); i += nthr) d32[off + i] = s32[first * (kTN / 2) + i];
+            const uint32_t n2 = tab_n(beta) / 2, off = level_off(beta) / 2;
+            const uint32_t *s32 =
+                reinterpret_cast<const uint32_t *>(W && n2 == kTabW / 2 ? a.node_tabs_w : a.node_tabs);
+            for (uint32_t i = tid; i < cnt * n2; i += nthr) d32[off + i] = s32[first * n2 + i];
         }
     };
     if (inv) stage_nodes(inv_t, a.inv_base + base);
@@ -384,7 +414,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         for (int j = 0; j < R; j++) data[rho<R>(wv, s_lo, t, j) * kTileDwords + lane] = v[j];
     };
     auto node_tab = [&](const uint16_t *tabs, uint32_t beta, uint32_t low_row) {
-        return tabs + ((G - (G >> beta)) + (low_row >> (beta + 1))) * kTN;
+        return tabs + level_off(beta) + (low_row >> (beta + 1)) * tab_n(beta);
+    };
+    // A product by the node table of level beta, in that level's format.
+    auto node_mul = [&](const uint16_t *tab, uint32_t beta, uint32_t x) {
+        if (MX && beta == 0) return gmul2(tab, x);
+        return gmul2n<W>(tab, x);
     };
     // In a stage over T bits at s_lo, T known at compile time, register j
     // holds local row (u mod 2^s_lo) | ((j mod 2^T) << s_lo) | ((u >> s_lo) << (s_lo + T))
@@ -395,8 +430,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     auto level_base = [&](const uint16_t *tabs, uint32_t s_lo, int T, int bs, int uidx) -> uint32_t {
         const uint32_t beta = s_lo + static_cast<uint32_t>(bs);
         const uint32_t u = wv * static_cast<uint32_t>(R >> T) + static_cast<uint32_t>(uidx);
-        const uint32_t node = (G - (G >> beta)) + ((u >> s_lo) << (T - 1 - bs));
-        uint32_t lb = __builtin_amdgcn_readfirstlane(lds_byte(tabs) + node * (kTN * 2));
+        const uint32_t node = (u >> s_lo) << (T - 1 - bs);  // within the level
+        uint32_t lb = __builtin_amdgcn_readfirstlane(lds_byte(tabs) + (level_off(beta) + node * tab_n(beta)) * 2);
         asm("" : "+s"(lb));
         return lb;
     };
@@ -413,47 +448,57 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     };
     // The butterflies of a level are software pipelined: the table reads of
     // product n are issued before product n - kPipe is combined.
-    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc) {
+    // fc: the stage starts at the pass's first level (s_lo == 0), whose
+    // tables are nibble tables in the mixed format.
+    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc, auto fc) {
         constexpr int T = decltype(tc)::value;
         constexpr bool inverse = decltype(ic)::value;
         constexpr int NU = R >> T;
-#pragma unroll
-        for (int step = 0; step < T; step++) {
-            const int bs = inverse ? step : T - 1 - step;
+        static_for<T>([&](auto stepc) {
+            constexpr int step = decltype(stepc)::value;
+            constexpr int bs = inverse ? step : T - 1 - step;
+            constexpr bool LW = W && !(MX && decltype(fc)::value && bs == 0);  // this level's format
+            constexpr uint32_t kLN = LW ? kTabW : kTab;
             uint32_t lb[NU];
 #pragma unroll
             for (int q = 0; q < NU; q++) lb[q] = level_base(tabs, s_lo, T, bs, q);
             // Butterfly n of the level: registers jn and jn + 2^bs.
             auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
-            auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kTN * 2); };
+            auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kLN * 2); };
             auto zero = [&](int j) { return decltype(zc)::value && off(j) == 0; };
-            Parts<W> pp[kPipe + 1];
+            Parts<LW> pp[kPipe + 1];
 #pragma unroll
             for (int n = 0; n < R / 2 + kPipe; n++) {
                 if (n < R / 2) {
                     const int j = jn(n);
                     if constexpr (inverse) v[j + (1 << bs)] ^= v[j];
-                    if (!zero(j)) gmul2_issue<W>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
+                    if (!zero(j)) gmul2_issue<LW>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 if (n >= kPipe) {
                     const int j = jn(n - kPipe);
-                    if (!zero(j)) v[j] ^= gmul2_combine<W>(pp[(n - kPipe) % (kPipe + 1)]);
+                    if (!zero(j)) v[j] ^= gmul2_combine<LW>(pp[(n - kPipe) % (kPipe + 1)]);
                     if constexpr (!inverse) v[j + (1 << bs)] ^= v[j];
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
-        }
+        });
     };
     // Full stages, and 3-bit stages (a 7-level pass), take the fast path.
     // The 8-level narrow-table form keeps one form of its stages: with both,
     // it needs more than the 128 registers of 4 waves per SIMD.
     auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, bool zs, auto ic) {
         auto run = [&](auto tc) {
-            if (zs && (W || GL == 0))
-                stage_fast(tabs, s_lo, tc, ic, std::true_type{});
-            else
-                stage_fast(tabs, s_lo, tc, ic, std::false_type{});
+            if (MX && s_lo == 0) {
+                if (zs)
+                    stage_fast(tabs, s_lo, tc, ic, std::true_type{}, std::bool_constant<MX>{});
+                else
+                    stage_fast(tabs, s_lo, tc, ic, std::false_type{}, std::bool_constant<MX>{});
+            } else if (zs && (W || GL == 0)) {
+                stage_fast(tabs, s_lo, tc, ic, std::true_type{}, std::false_type{});
+            } else {
+                stage_fast(tabs, s_lo, tc, ic, std::false_type{}, std::false_type{});
+            }
         };
         if (t == RB) {
             run(std::integral_constant<int, static_cast<int>(RB)>{});
@@ -478,7 +523,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
                 if (j & (1 << bs)) continue;
                 const uint16_t *tab = node_tab(inv_t, s_lo + bs, rho<R>(wv, s_lo, t, j));
                 v[j + (1 << bs)] ^= v[j];
-                v[j] ^= gmul2n<W>(tab, v[j + (1 << bs)]);
+                v[j] ^= node_mul(tab, s_lo + bs, v[j + (1 << bs)]);
             }
         }
     };
@@ -491,7 +536,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             for (int j = 0; j < R; j++) {
                 if (j & (1 << bs)) continue;
                 const uint16_t *tab = node_tab(fwd_t, s_lo + bs, rho<R>(wv, s_lo, t, j));
-                v[j] ^= gmul2n<W>(tab, v[j + (1 << bs)]);
+                v[j] ^= node_mul(tab, s_lo + bs, v[j + (1 << bs)]);
                 v[j + (1 << bs)] ^= v[j];
             }
         }
@@ -621,7 +666,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     }
 }
 
-template <int R, bool A4, uint32_t GL = 0, bool W = false, uint32_t FX = kFxDynamic>
+template <int R, bool A4, uint32_t GL = 0, bool W = false, uint32_t FX = kFxDynamic, bool MX = false>
 hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size_t lds, hipStream_t s) {
     const uint32_t G = 1u << a.g;
     const uint64_t tiles = (a.L + kTileDwords * 4 - 1) / (kTileDwords * 4);
@@ -639,9 +684,9 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W, FX>), 160 * 1024);
+    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W, MX, FX>), 160 * 1024);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL, W, FX>), grid, block, lds, s, a, tpb);
+    hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL, W, MX, FX>), grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
 
@@ -785,11 +830,14 @@ __global__ __launch_bounds__(256) void afft_scatter_kernel(const uint16_t *idx, 
 
 }  // namespace
 
-size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der, bool wide) {
+size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale_out, bool scale_der, bool wide,
+                           bool mixed) {
     const size_t G = size_t{1} << g, tab = kTab * 2, ntab = (wide ? kTabW : kTab) * 2;
+    // Mixed: the first level's G/2 tables in the nibble format.
+    const size_t nodes = mixed && wide ? (G / 2) * tab + (G / 2 - 1) * ntab : (G - 1) * ntab;
     size_t b = G * kTileDwords * 4;
-    if (flags & kAfftInv) b += (G - 1) * ntab;
-    if (flags & kAfftFwd) b += (G - 1) * ntab;
+    if (flags & kAfftInv) b += nodes;
+    if (flags & kAfftFwd) b += nodes;
     if ((flags & kAfftDeriv) && !(flags & kAfftDerivUnit)) b += 16 * tab;
     if (scale_in) b += G * tab;
     if (scale_out) b += G * tab;
@@ -814,6 +862,7 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     const size_t lds = afft_pass_lds_bytes(a.g, a.flags, sin, sout, sder, false);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const size_t lds_w = afft_pass_lds_bytes(a.g, a.flags, sin, sout, sder, true);
+    const size_t lds_m = afft_pass_lds_bytes(a.g, a.flags, sin, sout, sder, true, true);
     // Dword buffer instructions when every row the pass touches starts on a
     // multiple of 4 (row tables: as declared by the caller).
     auto al4 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; };
@@ -846,6 +895,13 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
             }
         }
         return launch_pass_r<16, true, 0, true>(a, D, count, lds_w, s);
+    }
+    // Mixed tables where they keep the nibble tables' waves per CU: three
+    // reads per element above the pass's first level instead of four.
+    if (a4 && a.g >= 4 && a.g < 8 && a.node_tabs_w && lds_m <= 160 * 1024 && waves_per_cu(lds_m) >= waves_per_cu(lds)) {
+        if (fx == (kFxInv | kFxFwd))
+            return launch_pass_r<16, true, 0, true, kFxInv | kFxFwd, true>(a, D, count, lds_m, s);
+        return launch_pass_r<16, true, 0, true, kFxDynamic, true>(a, D, count, lds_m, s);
     }
     if (a4) {
         switch (a.g) {
