@@ -292,6 +292,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     if (a.row_limit && (base >> b_lo) > ((a.row_limit - 1u) >> b_lo)) return;
     const bool inv = a.flags & kAfftInv, fwd = a.flags & kAfftFwd;
     const bool unit = a.flags & kAfftDerivUnit, der = unit || (a.flags & kAfftDeriv);
+    // A forward pass whose rows from row_limit on are not needed downstream,
+    // when row_limit is half the pass's row range (the group's rows with the
+    // pass's top bit set): after the top level those rows form a transform
+    // of their own, whose butterflies and stores are skipped.
+    const bool half = fwd && a.row_limit != 0 && a.row_limit == (1u << (b_lo + g - 1u)) && (gidx >> b_lo) == 0u;
     uint32_t *data = smem;
     // Node tables: level beta's G >> (beta + 1) tables of tab_n(beta) u16
     // entries from level_off(beta) on.
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
 #pragma unroll
         for (int j = 0; j < R; j++) {
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
-            uint8_t *dst = out_ptr(r);
+            uint8_t *dst = half && r >= a.row_limit ? nullptr : out_ptr(r);
             if constexpr (A4)
                 st_col4(dst, a.node_tabs, static_cast<uint32_t>(cb), L,
                         a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
@@ -450,9 +455,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // product n are issued before product n - kPipe is combined.
     // fc: the stage starts at the pass's first level (s_lo == 0), whose
     // tables are nibble tables in the mixed format.
-    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc, auto fc) {
+    // hc (forward, top stage): only the rows with the stage's top bit clear
+    // are needed; the top level's high outputs and every butterfly of the
+    // other half below it are skipped.
+    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc, auto fc, auto hc) {
         constexpr int T = decltype(tc)::value;
         constexpr bool inverse = decltype(ic)::value;
+        constexpr bool HALF = decltype(hc)::value;
         constexpr int NU = R >> T;
         static_for<T>([&](auto stepc) {
             constexpr int step = decltype(stepc)::value;
@@ -466,19 +475,22 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
             auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kLN * 2); };
             auto zero = [&](int j) { return decltype(zc)::value && off(j) == 0; };
+            auto dead = [&](int j) { return HALF && bs < T - 1 && (j & (1 << (T - 1))) != 0; };
             Parts<LW> pp[kPipe + 1];
 #pragma unroll
             for (int n = 0; n < R / 2 + kPipe; n++) {
                 if (n < R / 2) {
                     const int j = jn(n);
                     if constexpr (inverse) v[j + (1 << bs)] ^= v[j];
-                    if (!zero(j)) gmul2_issue<LW>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
+                    if (!zero(j) && !dead(j))
+                        gmul2_issue<LW>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 if (n >= kPipe) {
                     const int j = jn(n - kPipe);
-                    if (!zero(j)) v[j] ^= gmul2_combine<LW>(pp[(n - kPipe) % (kPipe + 1)]);
-                    if constexpr (!inverse) v[j + (1 << bs)] ^= v[j];
+                    if (!zero(j) && !dead(j)) v[j] ^= gmul2_combine<LW>(pp[(n - kPipe) % (kPipe + 1)]);
+                    if constexpr (!inverse)
+                        if (!dead(j) && !(HALF && bs == T - 1)) v[j + (1 << bs)] ^= v[j];
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -487,17 +499,26 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // Full stages, and 3-bit stages (a 7-level pass), take the fast path.
     // The 8-level narrow-table form keeps one form of its stages: with both,
     // it needs more than the 128 registers of 4 waves per SIMD.
-    auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, bool zs, auto ic) {
+    auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, bool zs, bool hs, auto ic) {
         auto run = [&](auto tc) {
+            auto go = [&](auto zc, auto fc) {
+                if constexpr (!decltype(ic)::value && (W || GL == 0)) {
+                    if (hs) {
+                        stage_fast(tabs, s_lo, tc, ic, zc, fc, std::true_type{});
+                        return;
+                    }
+                }
+                stage_fast(tabs, s_lo, tc, ic, zc, fc, std::false_type{});
+            };
             if (MX && s_lo == 0) {
                 if (zs)
-                    stage_fast(tabs, s_lo, tc, ic, std::true_type{}, std::bool_constant<MX>{});
+                    go(std::true_type{}, std::bool_constant<MX>{});
                 else
-                    stage_fast(tabs, s_lo, tc, ic, std::false_type{}, std::bool_constant<MX>{});
+                    go(std::false_type{}, std::bool_constant<MX>{});
             } else if (zs && (W || GL == 0)) {
-                stage_fast(tabs, s_lo, tc, ic, std::true_type{}, std::false_type{});
+                go(std::true_type{}, std::false_type{});
             } else {
-                stage_fast(tabs, s_lo, tc, ic, std::false_type{}, std::false_type{});
+                go(std::false_type{}, std::false_type{});
             }
         };
         if (t == RB) {
@@ -514,7 +535,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     };
     const uint32_t inv_pbase = a.inv_base + base, fwd_pbase = a.fwd_base + base;
     auto inv_stage = [&](uint32_t s_lo, uint32_t t) {
-        if (fast_stage(inv_t, s_lo, t, zero_stage(inv_pbase, s_lo, t), std::true_type{})) return;
+        if (fast_stage(inv_t, s_lo, t, zero_stage(inv_pbase, s_lo, t), false, std::true_type{})) return;
 #pragma unroll
         for (int bs = 0; bs < static_cast<int>(RB); bs++) {
             if (static_cast<uint32_t>(bs) >= t) break;
@@ -528,7 +549,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         }
     };
     auto fwd_stage = [&](uint32_t s_lo, uint32_t t) {
-        if (fast_stage(fwd_t, s_lo, t, zero_stage(fwd_pbase, s_lo, t), std::false_type{})) return;
+        // Half-output passes: below the top stage a wave's rows share the
+        // pass's top bit when a stage tile is one wave's (R >> t == 1), and
+        // the waves of the unneeded half have nothing to compute.
+        if (half && s_lo + t < g && (R >> t) == 1 && ((wv >> (g - 1u - t)) & 1u)) return;
+        if (fast_stage(fwd_t, s_lo, t, zero_stage(fwd_pbase, s_lo, t), half && s_lo + t == g, std::false_type{}))
+            return;
 #pragma unroll
         for (int bs = static_cast<int>(RB) - 1; bs >= 0; bs--) {
             if (static_cast<uint32_t>(bs) >= t) continue;
