@@ -54,7 +54,10 @@ def afft_table_reads(k: int, m: int, sl: int) -> tuple[int, int]:
     top pass (its upper 4 levels, 3 in a 7-level pass) of a transform that
     starts at point 0, the 2^B butterflies of level B on that node skip
     their product.  The encode's inverse and the decode's inverse and
-    forward start at 0; the encode's forward starts at k."""
+    forward start at 0; the encode's forward starts at k.  The decode's
+    forward over the high bits needs only the rows below k = N/2: below
+    its top level it runs on that half alone, where levels 15 and 14 lie on
+    node 0 and levels 13 and 12 on nodes 0-1 and 0-3."""
     assert k == m and k & (k - 1) == 0 and 2 * k == 1 << 16
     el = sl // 2
     per = 3
@@ -64,10 +67,11 @@ def afft_table_reads(k: int, m: int, sl: int) -> tuple[int, int]:
     enc = (8 * per + 2 * mid + 8 * per) * (k // 2) * el
     enc -= skipped(dk, 3) * per * el
     n = 2 * k
-    inv = 2 * 8 * per * (n // 2) * el + 2 * n * el * 4
-    fwd_hi = 8 * per * (n // 2) * el + n * el * 4
+    inv = 2 * 8 * per * (n // 2) * el + 2 * n * el * 4 - skipped(16, 4) * per * el
+    q = n // 4  # butterflies per level among the rows below k
+    fwd_hi = (q // 2 + 3 * q // 4 + 4 * q) * per * el + n * el * 4
     fwd_lo = 8 * per * (k // 2) * el + k * el * 4
-    dec = inv + fwd_hi + fwd_lo - 2 * skipped(16, 4) * per * el
+    dec = inv + fwd_hi + fwd_lo
     return enc, dec
 
 
