@@ -317,39 +317,65 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     uint16_t *sder_t = sout_t + (a.scale_out ? G * kTab : 0);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
 
-    // Stage the node tables of this group: level beta (pass bit b_lo+beta)
-    // uses G >> (beta+1) consecutive nodes.
+    // Table staging in 16-byte chunks, kStageIn chunks per thread in flight
+    // at once, so that one wait covers them all: chunk c of a staged block
+    // comes from src_chunk(c).
+    constexpr uint32_t kStageIn = 4;
+    auto stage_chunks = [&](uint16_t *dst, uint32_t chunks, auto src_chunk) {
+        uint4 *d = reinterpret_cast<uint4 *>(dst);
+        for (uint32_t c0 = 0; c0 < chunks; c0 += kStageIn * nthr) {
+            // Past the end, a thread loads and stores the last chunk again
+            // (the same value to the same place): no branch around the
+            // loads, which the compiler would otherwise sink each into its
+            // store's branch, one wait per chunk.
+            uint4 val[kStageIn];
+#pragma unroll
+            for (uint32_t q = 0; q < kStageIn; q++) val[q] = *src_chunk(min(c0 + q * nthr + tid, chunks - 1u));
+#pragma unroll
+            for (uint32_t q = 0; q < kStageIn; q++) d[min(c0 + q * nthr + tid, chunks - 1u)] = val[q];
+        }
+    };
+    // The node tables of this group: level beta (pass bit b_lo+beta) uses
+    // G >> (beta+1) consecutive nodes, all levels staged as one block.
+    // Table n of the block (in the pass's numbering, level by level) is of
+    // level g - 1 - floor(log2(G - 1 - n)).
     auto stage_nodes = [&](uint16_t *dst, uint32_t pbase) {
-        uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-        for (uint32_t beta = 0; beta < g; beta++) {
+        stage_chunks(dst, level_off(g) / 8u, [&](uint32_t c) {
+            uint32_t n, k;  // table of the block, chunk within it
+            if (MX && c < (G / 2) * (kTab / 8u)) {
+                n = c / (kTab / 8u);
+                k = c % (kTab / 8u);
+            } else {
+                const uint32_t c1 = MX ? c - (G / 2) * (kTab / 8u) : c;
+                n = (MX ? G / 2 : 0u) + c1 / (kTN / 8u);
+                k = c1 % (kTN / 8u);
+            }
+            const uint32_t beta = g - 1u - (31u - static_cast<uint32_t>(__builtin_clz(G - 1u - n)));
             const uint32_t B = b_lo + beta;
-            const uint32_t cnt = G >> (beta + 1);
-            const uint32_t first = (1u << (15u - B)) + (pbase >> (B + 1));
-            const uint32_t n2 = tab_n(beta) / 2, off = level_off(beta) / 2;
-            const uint32_t *s32 =
-                reinterpret_cast<const uint32_t *>(W && n2 == kTabW / 2 ? a.node_tabs_w : a.node_tabs);
-            for (uint32_t i = tid; i < cnt * n2; i += nthr) d32[off + i] = s32[first * n2 + i];
-        }
+            const uint32_t node = (1u << (15u - B)) + (pbase >> (B + 1)) + (n - (G - (G >> beta)));
+            const uint32_t tn = tab_n(beta);
+            const uint16_t *src = W && tn == kTabW ? a.node_tabs_w : a.node_tabs;
+            return reinterpret_cast<const uint4 *>(src + node * tn) + k;
+        });
     };
-    if (inv) stage_nodes(inv_t, a.inv_base + base);
-    if (fwd) stage_nodes(fwd_t, a.fwd_base + base);
-    if (der && !unit) {
-        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(a.deriv_tabs);
-        uint32_t *d32 = reinterpret_cast<uint32_t *>(der_t);
-        for (uint32_t i = tid; i < 16 * (kTab / 2); i += nthr) d32[i] = s32[i];
-    }
+    // Per-row tables of the group's rows (kTab entries, 8 chunks, per row).
     auto stage_rows = [&](uint16_t *dst, const uint16_t *src) {
-        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
-        uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-        for (uint32_t i = tid; i < G * (kTab / 2); i += nthr) {
-            const uint32_t r = base | ((i / (kTab / 2)) << b_lo);
-            d32[i] = s32[r * (kTab / 2) + i % (kTab / 2)];
-        }
+        stage_chunks(dst, G * (kTab / 8u), [&](uint32_t c) {
+            const uint32_t r = base | ((c / (kTab / 8u)) << b_lo);
+            return reinterpret_cast<const uint4 *>(src + r * kTab) + c % (kTab / 8u);
+        });
     };
-    if (a.scale_in) stage_rows(sin_t, a.scale_in);
-    if (a.scale_out) stage_rows(sout_t, a.scale_out);
-    if (a.scale_der) stage_rows(sder_t, a.scale_der);
-    __syncthreads();
+    auto stage_tables = [&]() {
+        if (inv) stage_nodes(inv_t, a.inv_base + base);
+        if (fwd) stage_nodes(fwd_t, a.fwd_base + base);
+        if (der && !unit) stage_chunks(der_t, 16u * (kTab / 8u), [&](uint32_t c) {
+            return reinterpret_cast<const uint4 *>(a.deriv_tabs) + c;
+        });
+        if (a.scale_in) stage_rows(sin_t, a.scale_in);
+        if (a.scale_out) stage_rows(sout_t, a.scale_out);
+        if (a.scale_der) stage_rows(sder_t, a.scale_der);
+        __syncthreads();
+    };
 
     const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
     const uint64_t L = a.L;
@@ -577,6 +603,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // ends in the first one's: either way the accumulator rows are added in
     // stage layout 0.
     const uint32_t k_der = 0;
+    stage_tables();
     if (t_first < t_end) load_raw(stage_lo(k_in), stage_t(k_in), t_first, nx);
     uint32_t ax[R];
     for (uint32_t tile = t_first; tile < t_end; tile++) {
