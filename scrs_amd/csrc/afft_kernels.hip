@@ -194,6 +194,36 @@ __device__ __forceinline__ uint32_t gmul2_combine(const Parts<W> &p) {
         return (p.r[0] ^ p.r[1] ^ p.r[2] ^ p.r[3]) | ((p.r[4] ^ p.r[5] ^ p.r[6] ^ p.r[7]) << 16);
 }
 
+// w AND m after taking w's upper half (SDWA operand select): the address
+// field of the high element in one instruction.  Pure ALU.
+__device__ __forceinline__ uint32_t and_hi(uint32_t m, uint32_t w) {
+    uint32_t r;
+    asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+        : "=v"(r)
+        : "s"(m), "v"(w));
+    return r;
+}
+// gmul2_issue (wide format) for a table at the compile-time LDS byte
+// address `off`: with no base register to merge, the six entry addresses
+// cost three shifts and six masks, the high element's through and_hi,
+// instead of twelve instructions.
+__device__ __forceinline__ void gmul2_issue_abs(Parts<true> &p, uint32_t off, uint32_t w) {
+    const uint32_t w2 = w << 1, w5 = w >> 5, w10 = w >> 10;
+    p.r[0] = lds_u16((w2 & 0x7Eu) + off);
+    p.r[1] = lds_u16((w5 & 0x3Eu) + off + 128u);
+    p.r[2] = lds_u16((w10 & 0x3Eu) + off + 192u);
+    p.r[3] = lds_u16(and_hi(0x7Eu, w2) + off);
+    p.r[4] = lds_u16(and_hi(0x3Eu, w5) + off + 128u);
+    p.r[5] = lds_u16(and_hi(0x3Eu, w10) + off + 192u);
+}
+// Byte offset of level beta's first node table in a pass's block of node
+// tables over G rows (tn u16 entries per table; mixed: the first level's
+// tables in the nibble format).
+constexpr uint32_t level_off_bytes(uint32_t G, uint32_t beta, bool mixed, uint32_t tn) {
+    const uint32_t below = G - (G >> beta);
+    return mixed ? (beta == 0 ? 0u : ((G / 2) * kTab + (below - G / 2) * kTabW) * 2u) : below * tn * 2u;
+}
+
 __device__ __forceinline__ uint32_t ld_col(const uint8_t *p, uint64_t cb, uint64_t L) {
     if (((reinterpret_cast<uintptr_t>(p) | L) & 3u) == 0) return *reinterpret_cast<const uint32_t *>(p + cb);
     uint32_t w = 0;
@@ -297,7 +327,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // pass's top bit set): after the top level those rows form a transform
     // of their own, whose butterflies and stores are skipped.
     const bool half = fwd && a.row_limit != 0 && a.row_limit == (1u << (b_lo + g - 1u)) && (gidx >> b_lo) == 0u;
-    uint32_t *data = smem;
     // Node tables: level beta's G >> (beta + 1) tables of tab_n(beta) u16
     // entries from level_off(beta) on.
     auto tab_n = [&](uint32_t beta) -> uint32_t { return (MX && beta == 0) ? kTab : kTN; };
@@ -308,13 +337,23 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         else
             return below * kTN;
     };
+    // LDS: the node tables first, so that with the pass's size and features
+    // fixed at compile time their addresses are constants (kAbs below), then
+    // the other tables, then the data tile.
     const uint32_t tabs_n = level_off(g);
-    uint16_t *inv_t = reinterpret_cast<uint16_t *>(smem + G * kTileDwords);
+    uint16_t *inv_t = reinterpret_cast<uint16_t *>(smem);
     uint16_t *fwd_t = inv_t + (inv ? tabs_n : 0);
     uint16_t *der_t = fwd_t + (fwd ? tabs_n : 0);
     uint16_t *sin_t = der_t + (der && !unit ? 16 * kTab : 0);
     uint16_t *sout_t = sin_t + (a.scale_in ? G * kTab : 0);
     uint16_t *sder_t = sout_t + (a.scale_out ? G * kTab : 0);
+    uint32_t *data = reinterpret_cast<uint32_t *>(sder_t + (a.scale_der ? G * kTab : 0));
+    // Top stages with compile-time table addresses: a pass of GL levels with
+    // a fixed feature set and wide tables.  kTabsByte(inverse): the LDS byte
+    // address of that direction's node tables.
+    constexpr bool kAbs = W && GL != 0 && FX != kFxDynamic;
+    constexpr uint32_t kGc = GL ? 1u << GL : 0u;
+    constexpr uint32_t kFwdByte = (FX & kFxInv) ? level_off_bytes(kGc, GL, MX, kTN) : 0u;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
 
     // Table staging in 16-byte chunks, kStageIn chunks per thread in flight
@@ -484,19 +523,29 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // hc (forward, top stage): only the rows with the stage's top bit clear
     // are needed; the top level's high outputs and every butterfly of the
     // other half below it are skipped.
-    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc, auto fc, auto hc) {
+    // tp (kAbs kernels): the pass's top stage, whose node of register j at
+    // level s_lo + bs is the level's table (j mod 2^T) >> (bs + 1) for every
+    // wave: the table addresses are compile-time constants.
+    auto stage_fast = [&](const uint16_t *tabs, uint32_t s_lo, auto tc, auto ic, auto zc, auto fc, auto hc, auto tp) {
         constexpr int T = decltype(tc)::value;
         constexpr bool inverse = decltype(ic)::value;
         constexpr bool HALF = decltype(hc)::value;
+        constexpr bool ABS = kAbs && decltype(tp)::value;
         constexpr int NU = R >> T;
         static_for<T>([&](auto stepc) {
             constexpr int step = decltype(stepc)::value;
             constexpr int bs = inverse ? step : T - 1 - step;
             constexpr bool LW = W && !(MX && decltype(fc)::value && bs == 0);  // this level's format
             constexpr uint32_t kLN = LW ? kTabW : kTab;
+            static_assert(!ABS || LW, "compile-time table addresses use the wide format");
+            // The level's first table's LDS byte address (ABS).
+            constexpr uint32_t kLevelByte =
+                ABS ? (inverse ? 0u : kFwdByte) + level_off_bytes(kGc, GL - T + bs, MX, kTN) : 0u;
             uint32_t lb[NU];
+            if constexpr (!ABS) {
 #pragma unroll
-            for (int q = 0; q < NU; q++) lb[q] = level_base(tabs, s_lo, T, bs, q);
+                for (int q = 0; q < NU; q++) lb[q] = level_base(tabs, s_lo, T, bs, q);
+            }
             // Butterfly n of the level: registers jn and jn + 2^bs.
             auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
             auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kLN * 2); };
@@ -508,8 +557,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
                 if (n < R / 2) {
                     const int j = jn(n);
                     if constexpr (inverse) v[j + (1 << bs)] ^= v[j];
-                    if (!zero(j) && !dead(j))
-                        gmul2_issue<LW>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
+                    if (!zero(j) && !dead(j)) {
+                        if constexpr (ABS)
+                            gmul2_issue_abs(pp[n % (kPipe + 1)], kLevelByte + off(j), v[j + (1 << bs)]);
+                        else
+                            gmul2_issue<LW>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 if (n >= kPipe) {
@@ -528,13 +581,23 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     auto fast_stage = [&](const uint16_t *tabs, uint32_t s_lo, uint32_t t, bool zs, bool hs, auto ic) {
         auto run = [&](auto tc) {
             auto go = [&](auto zc, auto fc) {
+                // The top stage of a kAbs kernel: compile-time table addresses.
+                auto go2 = [&](auto hc) {
+                    if constexpr (kAbs && !decltype(fc)::value) {
+                        if (s_lo + decltype(tc)::value == GL) {
+                            stage_fast(tabs, s_lo, tc, ic, zc, fc, hc, std::true_type{});
+                            return;
+                        }
+                    }
+                    stage_fast(tabs, s_lo, tc, ic, zc, fc, hc, std::false_type{});
+                };
                 if constexpr (!decltype(ic)::value && (W || GL == 0)) {
                     if (hs) {
-                        stage_fast(tabs, s_lo, tc, ic, zc, fc, std::true_type{});
+                        go2(std::true_type{});
                         return;
                     }
                 }
-                stage_fast(tabs, s_lo, tc, ic, zc, fc, std::false_type{});
+                go2(std::false_type{});
             };
             if (MX && s_lo == 0) {
                 if (zs)
@@ -551,7 +614,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             run(std::integral_constant<int, static_cast<int>(RB)>{});
             return true;
         }
-        if constexpr (GL == 0 && RB == 4) {
+        if constexpr ((GL == 0 || GL % 4 == 3) && RB == 4) {
             if (t == 3) {
                 run(std::integral_constant<int, 3>{});
                 return true;
@@ -952,8 +1015,8 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     // Mixed tables where they keep the nibble tables' waves per CU: three
     // reads per element above the pass's first level instead of four.
     if (a4 && a.g >= 4 && a.g < 8 && a.node_tabs_w && lds_m <= 160 * 1024 && waves_per_cu(lds_m) >= waves_per_cu(lds)) {
-        if (fx == (kFxInv | kFxFwd))
-            return launch_pass_r<16, true, 0, true, kFxInv | kFxFwd, true>(a, D, count, lds_m, s);
+        if (fx == (kFxInv | kFxFwd) && a.g == 7)
+            return launch_pass_r<16, true, 7, true, kFxInv | kFxFwd, true>(a, D, count, lds_m, s);
         return launch_pass_r<16, true, 0, true, kFxDynamic, true>(a, D, count, lds_m, s);
     }
     if (a4) {
