@@ -265,6 +265,39 @@ def test_full_domain_32768_32768_2048(torch_cuda):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("j", range(4, 16))
+def test_power_of_two_domains_round_trip(torch_cuda, afft_targeted, j):
+    """k = 2^j data symbols with m = k and m = k/2 repairs: every way the
+    transforms split into passes from 2^5 to 2^16 points, the fused encode,
+    and the erasure decode, whose forward over the top bits needs only the
+    rows below k when k is half the domain.  Encodes are checked against the
+    oracle up to 2^10 symbols, decodes against the data: half of all indices
+    erased at random, and (m = k) every data symbol erased."""
+    torch = torch_cuda
+    afft_targeted(0)  # every decode through the transforms
+    k = 1 << j
+    for m, L in ((k, 64), (k // 2, 34)):
+        p = Profile.resolve(AFFT, k, m, L)
+        data = torch.empty(k * L, dtype=torch.uint8, device="cuda")
+        fill_random_dev(data, data.numel(), 0x5EED + j * 3 + m)
+        rep = torch.empty(m * L, dtype=torch.uint8, device="cuda")
+        BatchEncoder(p).encode_batch_dev(data, rep, 1)
+        torch.cuda.synchronize()
+        if k <= 1024:
+            host = data.cpu().numpy()
+            assert np.array_equal(rep.cpu().numpy(), oracle.afft_encode(k, m, L, host)), (k, m, L)
+        dec = BatchDecoder(p)
+        rng = np.random.default_rng(j * 101 + m)
+        patterns = [np.sort(rng.choice(k + m, size=k, replace=False))]
+        if m == k:
+            patterns.append(np.arange(k, k + m))
+        for idx in patterns:
+            out = torch.full_like(data, 0xA5)
+            dec.decode_batch_dev(idx.astype(np.uint16)[None, :], data, rep, out, 1)
+            torch.cuda.synchronize()
+            assert torch.equal(out, data), (k, m, L, int(idx[0]))
+
+
 def erase_data(rng, k, m, r):
     """Received indices with exactly r data symbols missing: the first r
     received repairs replace them (every other repair is absent)."""
