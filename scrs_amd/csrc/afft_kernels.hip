@@ -800,8 +800,19 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     const uint32_t tpb = static_cast<uint32_t>(want < 1 ? 1 : (want > tiles ? tiles : want));
     const dim3 grid(static_cast<uint32_t>((tiles + tpb - 1) / tpb), static_cast<uint32_t>(groups), count);
     const dim3 block((G / R) * 64);
-    hipError_t err = lds_opt_in(reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W, MX, FX>), 160 * 1024);
+    const void *fn = reinterpret_cast<const void *>(&afft_pass_kernel<R, A4, GL, W, MX, FX>);
+    hipError_t err = lds_opt_in(fn, 160 * 1024);
     if (err != hipSuccess) return err;
+    if constexpr (W && GL != 0 && FX != kFxDynamic) {
+        // Constant top-stage table addresses assume the dynamic LDS block
+        // starts at address 0: the kernel declares no static LDS.
+        static const hipError_t placed = [fn] {
+            hipFuncAttributes attr{};
+            const hipError_t e = hipFuncGetAttributes(&attr, fn);
+            return e != hipSuccess ? e : (attr.sharedSizeBytes == 0 ? hipSuccess : hipErrorInvalidDeviceFunction);
+        }();
+        if (placed != hipSuccess) return placed;
+    }
     hipLaunchKernelGGL((afft_pass_kernel<R, A4, GL, W, MX, FX>), grid, block, lds, s, a, tpb);
     return hipGetLastError();
 }
