@@ -163,11 +163,14 @@ struct AfftEngine {
     // `count` codewords back to back: data c at data + c*k*L, repairs at
     // repairs + c*m*L.
     scrs_status encode(const uint8_t *data, uint8_t *repairs, uint64_t count, hipStream_t s);
-    // One codeword.  `known` lists the received wire indices (host, exactly
-    // k, distinct, validated).  Payloads from `ptrs` (device array, aligned
-    // with `indices`) or from the data / repair layout.
+    // One codeword.  `indices` lists the received wire indices (host,
+    // exactly k, distinct, validated); received64, when given, is the same
+    // set as a bitmap over the n indices (bit i of word i / 64).  Payloads
+    // from `ptrs` (device array, aligned with `indices`) or from the data /
+    // repair layout.
     scrs_status decode(const uint16_t *indices, const uint8_t *const *ptrs, const uint8_t *data_base,
-                       const uint8_t *repair_base, uint8_t *out, hipStream_t s);
+                       const uint8_t *repair_base, uint8_t *out, hipStream_t s,
+                       const uint64_t *received64 = nullptr);
 };
 
 }  // namespace scrs

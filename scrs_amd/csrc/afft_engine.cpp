@@ -169,7 +169,8 @@ scrs_status AfftEngine::init(const scrs_profile &p, bool for_decode) {
     }
     if (for_decode) {
         const std::vector<uint32_t> lw = afft_log_wht(D);
-        const uint64_t words = (N + 31) / 32;
+        // Whole 64-bit words, so a validation bitmap (64-bit words) copies in.
+        const uint64_t words = (N + 63) / 64 * 2;
         h_bits.assign(words, 0);
         if ((err = log_wht.ensure(N * 4)) != hipSuccess ||
             (err = hipMemcpy(log_wht.ptr, lw.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess ||
@@ -353,14 +354,25 @@ scrs_status AfftEngine::encode(const uint8_t *data, uint8_t *repairs, uint64_t c
 }
 
 scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *ptrs, const uint8_t *data_base,
-                               const uint8_t *repair_base, uint8_t *out, hipStream_t s) {
+                               const uint8_t *repair_base, uint8_t *out, hipStream_t s, const uint64_t *received64) {
     hipError_t err;
-    // Known-point bitmap through a pinned staging buffer reused per call.
-    std::fill(h_bits.begin(), h_bits.end(), 0u);
-    uint32_t missing = 0;
-    for (uint32_t t = 0; t < k; t++) h_bits[indices[t] >> 5] |= 1u << (indices[t] & 31u);
-    for (uint32_t x = 0; x < k; x++)
-        if (!((h_bits[x >> 5] >> (x & 31u)) & 1u)) missing++;
+    // Known-point bitmap (through a pinned staging buffer reused per call):
+    // the caller's validation bitmap when given (the same bits in 64-bit
+    // words, little-endian), else built from the indices.  The missing data
+    // symbols are k minus the known points below k.
+    const uint64_t n_words64 = (uint64_t{n} + 63) / 64;
+    if (received64) {
+        std::memcpy(h_bits.data(), received64, n_words64 * 8);
+        std::fill(reinterpret_cast<uint64_t *>(h_bits.data()) + n_words64,
+                  reinterpret_cast<uint64_t *>(h_bits.data() + h_bits.size()), uint64_t{0});
+    } else {
+        std::fill(h_bits.begin(), h_bits.end(), 0u);
+        for (uint32_t t = 0; t < k; t++) h_bits[indices[t] >> 5] |= 1u << (indices[t] & 31u);
+    }
+    uint32_t known_data = 0;
+    for (uint32_t w = 0; w < k / 32; w++) known_data += static_cast<uint32_t>(__builtin_popcount(h_bits[w]));
+    if (k % 32) known_data += static_cast<uint32_t>(__builtin_popcount(h_bits[k / 32] & ((1u << (k % 32)) - 1u)));
+    const uint32_t missing = k - known_data;
     if (missing > 0 && missing <= std::min(afft_targeted_max(), direct.rmax)) {
         auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
         const bool vec16 = !ptrs && L % 16 == 0 && al16(data_base) && al16(repair_base) && al16(out);

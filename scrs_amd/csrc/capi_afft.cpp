@@ -39,7 +39,7 @@ scrs_status scrs_afft_encode(void *engine, const uint8_t *data_dev, uint8_t *rep
 
 scrs_status scrs_afft_decode(void *engine, const uint16_t *indices, const uint8_t *const *ptrs_dev,
                              const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out_dev, uint64_t count,
-                             hipStream_t s) {
+                             hipStream_t s, const uint64_t *received64) {
     AfftEngine *e = engine_of(engine);
     std::lock_guard<std::mutex> lock(e->mu);
     hipError_t err = e->begin(s);
@@ -49,7 +49,7 @@ scrs_status scrs_afft_decode(void *engine, const uint16_t *indices, const uint8_
     for (uint64_t c = 0; c < count && r.code == SCRS_OK; c++)
         r = e->decode(indices + c * k, ptrs_dev ? ptrs_dev + c * k : nullptr,
                       data_base ? data_base + c * k * L : nullptr, repair_base ? repair_base + c * m * L : nullptr,
-                      out_dev + c * k * L, s);
+                      out_dev + c * k * L, s, count == 1 ? received64 : nullptr);
     err = e->finish(s);
     if (r.code == SCRS_OK && err != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     return r;

@@ -477,8 +477,9 @@ static scrs_status decode_batch(scrs_batch_decoder *dec, const uint16_t *indices
     }
     if (dec->prof.engine == SCRS_ENGINE_AFFT) {
         DeviceGuard g(dec->device);
+        // With one codeword, the validation's bitmap is its received set.
         return scrs_afft_decode(dec->afft, indices, payload_ptrs_dev, data_dev, repairs_dev, out_dev, count,
-                                static_cast<hipStream_t>(stream));
+                                static_cast<hipStream_t>(stream), count == 1 ? dec->seen.data() : nullptr);
     }
     DeviceGuard g(dec->device);
     // Pointer-table payloads may be unaligned: the 16-byte path is taken

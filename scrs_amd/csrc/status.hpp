@@ -25,6 +25,8 @@ scrs_status scrs_afft_encode(void *engine, const uint8_t *data_dev, uint8_t *rep
                              hipStream_t s);
 // `count` codewords; codeword c's k received indices at indices + c*k (host,
 // validated), payloads from ptrs_dev + c*k or the data / repair layout.
+// received64 (optional, count == 1): the validation's bitmap of the received
+// indices, bit i of word i / 64, over the n indices.
 scrs_status scrs_afft_decode(void *engine, const uint16_t *indices, const uint8_t *const *ptrs_dev,
                              const uint8_t *data_base, const uint8_t *repair_base, uint8_t *out_dev, uint64_t count,
-                             hipStream_t s);
+                             hipStream_t s, const uint64_t *received64 = nullptr);
