@@ -43,6 +43,10 @@ enum : uint32_t {
     kFxAux = 1u << 8,      // derivative accumulator rows
     kFxInRows = 1u << 9,   // input row table
     kFxOutRows = 1u << 10, // output row table
+    // Input row scale whose presence stays a run-time test: with it fixed,
+    // the compiler schedules the decode's first pass into more registers
+    // than 4 waves per SIMD allow, and its prefetched rows spill.
+    kFxSinRt = 1u << 11,
     kFxDynamic = ~0u,      // features read from the arguments
 };
 
@@ -304,7 +308,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             else
                 ptr = nullptr;
         };
-        fix(a.scale_in, FX & kFxSin);
+        if (!(FX & kFxSinRt)) fix(a.scale_in, FX & kFxSin);
         fix(a.scale_out, FX & kFxSout);
         fix(a.scale_der, FX & kFxSder);
         fix(a.copy_rows, FX & kFxCopy);
@@ -1006,12 +1010,13 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     const uint32_t fx = pass_features(a);
     if (a4 && a.g >= 4 && a.node_tabs_w && lds_w <= 160 * 1024 && waves_per_cu(lds_w) >= waves_per_cu(lds)) {
         if (a.g == 8) {
-            // The feature sets of the encode and decode pass chains, except
-            // the decode's first pass (input row table, copies and input
-            // scale): fixed, its prefetched rows spill.
+            // The feature sets of the encode and decode pass chains.
             switch (fx) {
                 case kFxInv: return launch_pass_r<16, true, 8, true, kFxInv>(a, D, count, lds_w, s);
                 case kFxFwd: return launch_pass_r<16, true, 8, true, kFxFwd>(a, D, count, lds_w, s);
+                case kFxInv | kFxInRows | kFxCopy | kFxSin:
+                    return launch_pass_r<16, true, 8, true, kFxInv | kFxInRows | kFxCopy | kFxSinRt>(a, D, count, lds_w,
+                                                                                                   s);
                 case kFxInv | kFxSout: return launch_pass_r<16, true, 8, true, kFxInv | kFxSout>(a, D, count, lds_w, s);
                 case kFxUnit: return launch_pass_r<16, true, 8, true, kFxUnit>(a, D, count, lds_w, s);
                 case kFxUnit | kFxFwd | kFxSder | kFxAux:
