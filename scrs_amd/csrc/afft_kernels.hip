@@ -670,8 +670,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // ends in the first one's: either way the accumulator rows are added in
     // stage layout 0.
     const uint32_t k_der = 0;
-    stage_tables();
+    // The first tile's rows load before the tables stage, so that both are
+    // in flight at once (the row loads depend on no staged table).
     if (t_first < t_end) load_raw(stage_lo(k_in), stage_t(k_in), t_first, nx);
+    stage_tables();
     uint32_t ax[R];
     for (uint32_t tile = t_first; tile < t_end; tile++) {
         cb = (static_cast<uint64_t>(tile) * kTileDwords + lane) * 4u;
