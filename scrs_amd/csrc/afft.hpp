@@ -54,8 +54,7 @@ struct AfftPassArgs {
     // Per-row tables applied to the derivative (after the accumulator,
     // before the forward levels).
     const uint16_t *scale_der = nullptr;
-    const uint16_t *node_tabs = nullptr;   // 65536 node tables
-    const uint16_t *node_tabs_w = nullptr; // the same in the wide format (null: nibble tables only)
+    const uint16_t *node_fac = nullptr;    // 65536 node factors (the pass's node tables are built from them)
     const uint16_t *deriv_tabs = nullptr;  // 16 derivative tables
     uint64_t L = 0;
     uint32_t b_lo = 0, g = 0;              // pass bits [b_lo, b_lo + g)
@@ -76,7 +75,6 @@ size_t afft_pass_lds_bytes(uint32_t g, uint32_t flags, bool scale_in, bool scale
 hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s);
 // Wide tables: out[i*128 + n] = c[i]*n for n < 64, out[i*128 + 64 + n] =
 // c[i]*(n << 6) and out[i*128 + 96 + n] = c[i]*(n << 11) for n < 32.
-hipError_t launch_afft_tables_wide(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s);
 
 // Locator constants for the point set given by the bitmap `known_bits`
 // (bit x of word x/32) over [0, 2^D): the per-row scale of the decode.
@@ -111,7 +109,10 @@ hipError_t launch_afft_scatter_ptrs(const uint16_t *indices, const uint8_t *cons
 
 // Per-device constant tables.
 struct AfftDeviceTables {
-    DevBuf node_tabs, node_tabs_w, deriv_tabs, exp;
+    DevBuf deriv_tabs, exp;
+    // The factor of every tree node (65536 u16), from which the pass kernel
+    // builds its node tables in LDS.
+    DevBuf node_fac;
     // Row tables of T(q) = prod over the bits b of q of the derivative
     // factor d_b, and of 1/T(q), for q < 65536 (kAfftDerivUnit).
     DevBuf deriv_scale, deriv_unscale;

@@ -109,17 +109,11 @@ hipError_t afft_device_tables(const AfftDeviceTables **out) {
         return hipStreamSynchronize(nullptr);
     };
     if ((err = consts.ensure(65536 * 2)) == hipSuccess &&
-        (err = t->node_tabs.ensure(65536ull * 128)) == hipSuccess &&
         (err = t->deriv_tabs.ensure(kAfftMaxLog * 128)) == hipSuccess &&
         (err = t->exp.ensure(kAfftOrder * 2)) == hipSuccess &&
         (err = hipMemcpy(t->exp.ptr, m.exp.data(), kAfftOrder * 2, hipMemcpyHostToDevice)) == hipSuccess &&
-        (err = hipMemcpy(consts.ptr, m.factors.data(), 65536 * 2, hipMemcpyHostToDevice)) == hipSuccess &&
-        (err = launch_afft_tables(consts.as<uint16_t>(), 65536, t->node_tabs.as<uint16_t>(), nullptr)) ==
-            hipSuccess &&
-        (err = t->node_tabs_w.ensure(65536ull * 256)) == hipSuccess &&
-        (err = launch_afft_tables_wide(consts.as<uint16_t>(), 65536, t->node_tabs_w.as<uint16_t>(), nullptr)) ==
-            hipSuccess &&
-        (err = hipStreamSynchronize(nullptr)) == hipSuccess &&
+        (err = t->node_fac.ensure(65536 * 2)) == hipSuccess &&
+        (err = hipMemcpy(t->node_fac.ptr, m.factors.data(), 65536 * 2, hipMemcpyHostToDevice)) == hipSuccess &&
         (err = hipMemcpy(consts.ptr, d.data(), kAfftMaxLog * 2, hipMemcpyHostToDevice)) == hipSuccess &&
         (err = launch_afft_tables(consts.as<uint16_t>(), kAfftMaxLog, t->deriv_tabs.as<uint16_t>(), nullptr)) ==
             hipSuccess &&
@@ -216,8 +210,7 @@ scrs_status AfftEngine::encode(const uint8_t *data, uint8_t *repairs, uint64_t c
         enc_cap = chunk;
     }
     AfftPassArgs base;
-    base.node_tabs = dev->node_tabs.as<uint16_t>();
-    base.node_tabs_w = dev->node_tabs_w.as<uint16_t>();
+    base.node_fac = dev->node_fac.as<uint16_t>();
     base.deriv_tabs = dev->deriv_tabs.as<uint16_t>();
     base.L = L;
     uint8_t *w1 = work.as<uint8_t>(), *w2 = work2.as<uint8_t>();
@@ -407,8 +400,7 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
         return st(SCRS_ERR_DEVICE, err);
     if (ptrs && (err = hipStreamSynchronize(s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
     AfftPassArgs base;
-    base.node_tabs = dev->node_tabs.as<uint16_t>();
-    base.node_tabs_w = dev->node_tabs_w.as<uint16_t>();
+    base.node_fac = dev->node_fac.as<uint16_t>();
     base.deriv_tabs = dev->deriv_tabs.as<uint16_t>();
     base.L = L;
     // Row tables point into the data, repair and output buffers at multiples

@@ -13,7 +13,8 @@
 // index, over a 256-byte column tile; each lane owns one dword column.  The
 // g levels run in stages of up to 4 bits: a thread keeps the 16 rows of one
 // stage tile in registers and tiles are exchanged through LDS between
-// stages.  The node tables of the pass are staged in LDS once.
+// stages.  The node tables of the pass are built in LDS once, from the
+// nodes' factors.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +31,20 @@ namespace {
 constexpr uint32_t kTileDwords = 64;  // 256-byte column tile
 constexpr uint32_t kTab = 64;         // u16 entries per constant table
 constexpr uint32_t kTabW = 128;       // u16 entries per wide constant table
+// GF(2^16) as GF(2^8)[u] / (u^2 + u + 0x20), GF(2^8) modulo 0x11B; bit b
+// of an element is x^b (b < 8) or u x^(b-8).
+// v * x: both bytes times x in GF(2^8).
+__device__ __forceinline__ uint32_t xtime16(uint32_t v) {
+    return ((v << 1) & 0xFEFEu) ^ (((v >> 7) & 0x0101u) * 0x1Bu);
+}
+// v * u with u^2 = u + 0x20: (a + b u) u = 0x20 b + (a + b) u.
+__device__ __forceinline__ uint32_t times_u(uint32_t v) {
+    const uint32_t a = v & 0xffu, b = v >> 8;
+    uint32_t lo = b;
+#pragma unroll
+    for (int i = 0; i < 5; i++) lo = xtime16(lo);
+    return lo | ((a ^ b) << 8);
+}
 // Pass features, for kernels specialised on them.
 enum : uint32_t {
     kFxInv = 1u << 0,      // inverse levels
@@ -274,7 +289,7 @@ __device__ __forceinline__ uint32_t rho(uint32_t wv, uint32_t s_lo, uint32_t t, 
 }
 
 // One workgroup owns a row group and `tpb` consecutive column tiles: the
-// node tables are staged once and the tiles run in turn, the next tile's
+// node tables are built once and the tiles run in turn, the next tile's
 // rows loading while the current one computes.
 // GL: the pass's level count fixed at compile time (0: taken from the
 // arguments), so a full pass's stage layouts and table offsets fold into
@@ -360,9 +375,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     constexpr uint32_t kFwdByte = (FX & kFxInv) ? level_off_bytes(kGc, GL, MX, kTN) : 0u;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
 
-    // Table staging in 16-byte chunks, kStageIn chunks per thread in flight
-    // at once, so that one wait covers them all: chunk c of a staged block
-    // comes from src_chunk(c).
+    // Row and derivative table staging in 16-byte chunks, kStageIn chunks
+    // per thread in flight at once, so that one wait covers them all: chunk
+    // c of a staged block comes from src_chunk(c).
     constexpr uint32_t kStageIn = 4;
     auto stage_chunks = [&](uint16_t *dst, uint32_t chunks, auto src_chunk) {
         uint4 *d = reinterpret_cast<uint4 *>(dst);
@@ -378,28 +393,86 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             for (uint32_t q = 0; q < kStageIn; q++) d[min(c0 + q * nthr + tid, chunks - 1u)] = val[q];
         }
     };
-    // The node tables of this group: level beta (pass bit b_lo+beta) uses
-    // G >> (beta+1) consecutive nodes, all levels staged as one block.
-    // Table n of the block (in the pass's numbering, level by level) is of
-    // level g - 1 - floor(log2(G - 1 - n)).
-    auto stage_nodes = [&](uint16_t *dst, uint32_t pbase) {
-        stage_chunks(dst, level_off(g) / 8u, [&](uint32_t c) {
-            uint32_t n, k;  // table of the block, chunk within it
-            if (MX && c < (G / 2) * (kTab / 8u)) {
-                n = c / (kTab / 8u);
-                k = c % (kTab / 8u);
-            } else {
-                const uint32_t c1 = MX ? c - (G / 2) * (kTab / 8u) : c;
-                n = (MX ? G / 2 : 0u) + c1 / (kTN / 8u);
-                k = c1 % (kTN / 8u);
-            }
+    // The node tables of this group, built in LDS from the nodes' factors.
+    // Level beta (pass bit b_lo+beta) uses G >> (beta+1) consecutive nodes,
+    // all levels as one block per direction; table n of the block (in the
+    // pass's numbering, level by level) is of level
+    // g - 1 - floor(log2(G - 1 - n)).  A table holds the products of its
+    // factor c with every value of each address field, and c*v is the XOR
+    // of c*2^b over the bits b of v.  A work item is a 32-entry quarter of
+    // a wide table (the halves of its 6-bit field, its two 5-bit fields) or
+    // half of a nibble table (two nibble fields).  The quarter is the outer
+    // index, so that a wave's items are of one kind when a block has 64
+    // tables or more.
+    // A thread has at most two items (4 * 2G items over G / R * 64 >= 4G
+    // threads), items tid and tid + nthr.  Their factors load before the
+    // first tile's rows, so that the tables build while the rows arrive.
+    const uint32_t lp = g + ((inv && fwd) ? 1u : 0u), per = 1u << lp;  // items per quarter
+    // Node of item `item`, or ~0u for no table part.
+    auto item_node = [&](uint32_t item) -> uint32_t {
+        const uint32_t q = item >> lp, rest = item & (per - 1u), n = rest & (G - 1u);
+        if (!(inv || fwd) || item >= 4u * per || n == G - 1u) return ~0u;
+        const uint32_t beta = g - 1u - (31u - static_cast<uint32_t>(__builtin_clz(G - 1u - n)));
+        if (tab_n(beta) != kTabW && q >= 2u) return ~0u;
+        const uint32_t B = b_lo + beta, first = G - (G >> beta);
+        const uint32_t pbase = ((!inv || rest >= G) ? a.fwd_base : a.inv_base) + base;
+        return (1u << (15u - B)) + (pbase >> (B + 1)) + (n - first);
+    };
+    uint32_t fac[2];
+    auto load_factors = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++) {
+            const uint32_t node = item_node(tid + k * nthr);
+            fac[k] = a.node_fac[node == ~0u ? 0u : node];
+        }
+    };
+    auto build_nodes = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++) {
+            const uint32_t item = tid + k * nthr;
+            if (item_node(item) == ~0u) continue;
+            const uint32_t q = item >> lp, rest = item & (per - 1u), n = rest & (G - 1u);
             const uint32_t beta = g - 1u - (31u - static_cast<uint32_t>(__builtin_clz(G - 1u - n)));
-            const uint32_t B = b_lo + beta;
-            const uint32_t node = (1u << (15u - B)) + (pbase >> (B + 1)) + (n - (G - (G >> beta)));
-            const uint32_t tn = tab_n(beta);
-            const uint16_t *src = W && tn == kTabW ? a.node_tabs_w : a.node_tabs;
-            return reinterpret_cast<const uint4 *>(src + node * tn) + k;
-        });
+            const bool wide = tab_n(beta) == kTabW, use_fwd = !inv || rest >= G;
+            const uint32_t first = G - (G >> beta);
+            uint32_t cb2[16];  // c * 2^b
+            cb2[0] = fac[k];
+#pragma unroll
+            for (int b = 1; b < 8; b++) cb2[b] = xtime16(cb2[b - 1]);
+            cb2[8] = times_u(fac[k]);
+#pragma unroll
+            for (int b = 9; b < 16; b++) cb2[b] = xtime16(cb2[b - 1]);
+            // Entry h*16 + m of the item (m < 16) is f[h] + sum over the bits
+            // b of m of x[h][b].
+            uint32_t x[2][4], f[2];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                if (wide) {
+                    x[0][b] = x[1][b] = q < 2u ? cb2[b] : (q == 2u ? cb2[6 + b] : cb2[11 + b]);
+                } else {
+                    x[0][b] = q == 0u ? cb2[b] : cb2[8 + b];
+                    x[1][b] = q == 0u ? cb2[4 + b] : cb2[12 + b];
+                }
+            }
+            f[0] = wide && q == 1u ? cb2[5] : 0u;
+            f[1] = wide ? f[0] ^ (q < 2u ? cb2[4] : (q == 2u ? cb2[10] : cb2[15])) : 0u;
+            uint4 *d = reinterpret_cast<uint4 *>((use_fwd ? fwd_t : inv_t) + level_off(beta) + (n - first) * tab_n(beta) +
+                                                 32u * q);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                uint32_t e[8];  // entries 2i, packed with entry 2i + 1
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    uint32_t v = f[h];
+#pragma unroll
+                    for (int b = 1; b < 4; b++)
+                        if ((i >> (b - 1)) & 1) v ^= x[h][b];
+                    e[i] = v | ((v ^ x[h][0]) << 16);
+                }
+                d[2 * h] = make_uint4(e[0], e[1], e[2], e[3]);
+                d[2 * h + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+            }
+        }
     };
     // Per-row tables of the group's rows (kTab entries, 8 chunks, per row).
     auto stage_rows = [&](uint16_t *dst, const uint16_t *src) {
@@ -409,8 +482,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         });
     };
     auto stage_tables = [&]() {
-        if (inv) stage_nodes(inv_t, a.inv_base + base);
-        if (fwd) stage_nodes(fwd_t, a.fwd_base + base);
+        if (inv || fwd) build_nodes();
         if (der && !unit) stage_chunks(der_t, 16u * (kTab / 8u), [&](uint32_t c) {
             return reinterpret_cast<const uint4 *>(a.deriv_tabs) + c;
         });
@@ -446,7 +518,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             const uint8_t *src = in_ptr(r);
             if constexpr (A4)
-                dst[j] = ld_col4(src, a.node_tabs, static_cast<uint32_t>(cbx), L);
+                dst[j] = ld_col4(src, a.node_fac, static_cast<uint32_t>(cbx), L);
             else
                 dst[j] = (src && cbx < L) ? ld_col(src, cbx, L) : 0u;
         }
@@ -460,7 +532,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             if (a.copy_rows) {
                 uint8_t *cp = load_uniform(a.copy_rows + r);
                 if constexpr (A4)
-                    st_col4(cp, a.node_tabs, static_cast<uint32_t>(cb), L, v[j]);
+                    st_col4(cp, a.node_fac, static_cast<uint32_t>(cb), L, v[j]);
                 else if (cp && valid)
                     st_col(cp, cb, L, v[j]);
             }
@@ -473,7 +545,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             const uint32_t p = rho<R>(wv, s_lo, t, j), r = base | (p << b_lo);
             uint8_t *dst = half && r >= a.row_limit ? nullptr : out_ptr(r);
             if constexpr (A4)
-                st_col4(dst, a.node_tabs, static_cast<uint32_t>(cb), L,
+                st_col4(dst, a.node_fac, static_cast<uint32_t>(cb), L,
                         a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
             else if (dst && valid)
                 st_col(dst, cb, L, a.scale_out ? gmul2(sout_t + p * kTab, v[j]) : v[j]);
@@ -670,8 +742,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // ends in the first one's: either way the accumulator rows are added in
     // stage layout 0.
     const uint32_t k_der = 0;
-    // The first tile's rows load before the tables stage, so that both are
-    // in flight at once (the row loads depend on no staged table).
+    // The node factors load first, then the first tile's rows, and the
+    // tables build and stage while the rows arrive (no row load depends on
+    // a table).
+    load_factors();
     if (t_first < t_end) load_raw(stage_lo(k_in), stage_t(k_in), t_first, nx);
     stage_tables();
     uint32_t ax[R];
@@ -686,7 +760,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
                 const uint32_t p = rho<R>(wv, stage_lo(k_der), stage_t(k_der), j);
                 const uint8_t *row = a.aux_base + z * a.aux_cw_stride + static_cast<uint64_t>(base | (p << b_lo)) * L;
                 if constexpr (A4)
-                    ax[j] = ld_col4(row, a.node_tabs, static_cast<uint32_t>(cb), L);
+                    ax[j] = ld_col4(row, a.node_fac, static_cast<uint32_t>(cb), L);
                 else
                     ax[j] = valid ? ld_col(row, cb, L) : 0u;
             }
@@ -799,7 +873,7 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
     if (a.row_limit)
         groups = std::min<uint64_t>(groups, ((uint64_t{(a.row_limit - 1u) >> (a.b_lo + a.g)}) + 1) << a.b_lo);
     // Tiles per workgroup: enough workgroups to fill every CU to its LDS
-    // capacity, no more, so each group's tables are staged as few times as
+    // capacity, no more, so each group's tables are built as few times as
     // possible.
     const uint64_t per_cu = lds ? std::max<uint64_t>(1, (160u * 1024u) / lds) : 8;
     const uint64_t want = tiles * groups * count / (256 * per_cu);
@@ -825,16 +899,6 @@ hipError_t launch_pass_r(const AfftPassArgs &a, uint32_t D, uint32_t count, size
 
 // ---- constant tables -------------------------------------------------------
 
-__device__ __forceinline__ uint32_t xtime8(uint32_t x) { return ((x << 1) ^ ((x & 0x80u) ? 0x1Bu : 0u)) & 0xffu; }
-__device__ __forceinline__ uint32_t xtime16(uint32_t v) { return xtime8(v & 0xffu) | (xtime8(v >> 8) << 8); }
-// v * u with u^2 = u + 0x20: (a + b u) u = 0x20 b + (a + b) u.
-__device__ __forceinline__ uint32_t times_u(uint32_t v) {
-    const uint32_t a = v & 0xffu, b = v >> 8;
-    uint32_t lo = b;
-#pragma unroll
-    for (int i = 0; i < 5; i++) lo = xtime8(lo);
-    return lo | ((a ^ b) << 8);
-}
 
 __global__ __launch_bounds__(256) void afft_tables_kernel(const uint16_t *c, uint32_t count, uint16_t *out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -860,31 +924,6 @@ __global__ __launch_bounds__(256) void afft_tables_kernel(const uint16_t *c, uin
 }
 
 // Wide tables (the 6 + 5 + 5 split of gmulw).
-__global__ __launch_bounds__(256) void afft_tables_wide_kernel(const uint16_t *c, uint32_t count, uint16_t *out) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= count) return;
-    uint32_t basis[16];
-    basis[0] = c[i];
-    for (int b = 1; b < 8; b++) basis[b] = xtime16(basis[b - 1]);
-    basis[8] = times_u(c[i]);
-    for (int b = 9; b < 16; b++) basis[b] = xtime16(basis[b - 1]);
-    uint32_t *o = reinterpret_cast<uint32_t *>(out + static_cast<uint64_t>(i) * kTabW);
-    // Entries [first, first + 2^bits): the products with n << bit0.
-    auto part = [&](uint32_t first, int bit0, int bits) {
-        for (int n = 0; n < (1 << bits); n += 2) {
-            uint32_t e0 = 0, e1 = 0;
-            for (int b = 0; b < bits; b++) {
-                if ((n >> b) & 1) e0 ^= basis[bit0 + b];
-                if (((n + 1) >> b) & 1) e1 ^= basis[bit0 + b];
-            }
-            o[(first + n) / 2] = e0 | (e1 << 16);
-        }
-    };
-    part(0, 0, 6);
-    part(64, 6, 5);
-    part(96, 11, 5);
-}
-
 // ---- locator (afft/decoder.rs:672-771) -------------------------------------
 
 __device__ __forceinline__ uint32_t addmod(uint32_t x, uint32_t y) {
@@ -991,6 +1030,7 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     if (a.g == 0 || a.g > 8 || a.b_lo + a.g > D || D > kAfftMaxLog || count == 0 || count > 65535 || a.L == 0)
         return hipErrorInvalidValue;
     if (a.scale_der && !(a.flags & (kAfftDeriv | kAfftDerivUnit))) return hipErrorInvalidValue;
+    if ((a.flags & (kAfftInv | kAfftFwd)) && !a.node_fac) return hipErrorInvalidValue;
     const bool sin = a.scale_in != nullptr, sout = a.scale_out != nullptr, sder = a.scale_der != nullptr;
     const size_t lds = afft_pass_lds_bytes(a.g, a.flags, sin, sout, sder, false);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -1010,7 +1050,7 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
         return std::min<size_t>(16, wgs * waves);
     };
     const uint32_t fx = pass_features(a);
-    if (a4 && a.g >= 4 && a.node_tabs_w && lds_w <= 160 * 1024 && waves_per_cu(lds_w) >= waves_per_cu(lds)) {
+    if (a4 && a.g >= 4 && lds_w <= 160 * 1024 && waves_per_cu(lds_w) >= waves_per_cu(lds)) {
         if (a.g == 8) {
             // The feature sets of the encode and decode pass chains.
             switch (fx) {
@@ -1032,7 +1072,7 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
     }
     // Mixed tables where they keep the nibble tables' waves per CU: three
     // reads per element above the pass's first level instead of four.
-    if (a4 && a.g >= 4 && a.g < 8 && a.node_tabs_w && lds_m <= 160 * 1024 && waves_per_cu(lds_m) >= waves_per_cu(lds)) {
+    if (a4 && a.g >= 4 && a.g < 8 && lds_m <= 160 * 1024 && waves_per_cu(lds_m) >= waves_per_cu(lds)) {
         if (fx == (kFxInv | kFxFwd) && a.g == 7)
             return launch_pass_r<16, true, 7, true, kFxInv | kFxFwd, true>(a, D, count, lds_m, s);
         return launch_pass_r<16, true, 0, true, kFxDynamic, true>(a, D, count, lds_m, s);
@@ -1059,12 +1099,6 @@ hipError_t launch_afft_pass(const AfftPassArgs &a, uint32_t D, uint32_t count, h
 hipError_t launch_afft_tables(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(afft_tables_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c, count, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_afft_tables_wide(const uint16_t *c, uint32_t count, uint16_t *out, hipStream_t s) {
-    if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(afft_tables_wide_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c, count, out);
     return hipGetLastError();
 }
 

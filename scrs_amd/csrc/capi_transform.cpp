@@ -44,8 +44,7 @@ hipError_t run_passes(const scrs_transform_plan *p, uint8_t *rows, uint64_t L, u
         b += g[i];
     }
     AfftPassArgs base;
-    base.node_tabs = p->dev->node_tabs.as<uint16_t>();
-    base.node_tabs_w = p->dev->node_tabs_w.as<uint16_t>();
+    base.node_fac = p->dev->node_fac.as<uint16_t>();
     base.deriv_tabs = p->dev->deriv_tabs.as<uint16_t>();
     base.L = L;
     base.in_base = rows;
