@@ -47,12 +47,12 @@ struct AfftPassArgs {
     uint64_t out_first = 0, out_limit = 0, out_cw_stride = 0;
     // Loaded rows are also copied, unscaled, to copy_rows[r] when non-null.
     uint8_t *const *copy_rows = nullptr;
-    // Per-row constant tables (64 u16 each, row r at r*64) applied after
-    // the load / before the store.
+    // Per-row constant factors (row r's at [r]) multiplied in after the
+    // load / before the store; the pass builds their product tables in LDS.
     const uint16_t *scale_in = nullptr;
     const uint16_t *scale_out = nullptr;
-    // Per-row tables applied to the derivative (after the accumulator,
-    // before the forward levels).
+    // Per-row factors multiplied into the derivative (after the
+    // accumulator, before the forward levels).
     const uint16_t *scale_der = nullptr;
     const uint16_t *node_fac = nullptr;    // 65536 node factors (the pass's node tables are built from them)
     const uint16_t *deriv_tabs = nullptr;  // 16 derivative tables
@@ -113,8 +113,8 @@ struct AfftDeviceTables {
     // The factor of every tree node (65536 u16), from which the pass kernel
     // builds its node tables in LDS.
     DevBuf node_fac;
-    // Row tables of T(q) = prod over the bits b of q of the derivative
-    // factor d_b, and of 1/T(q), for q < 65536 (kAfftDerivUnit).
+    // T(q) = prod over the bits b of q of the derivative factor d_b, and
+    // 1/T(q), for q < 65536 (kAfftDerivUnit).
     DevBuf deriv_scale, deriv_unscale;
 };
 hipError_t afft_device_tables(const AfftDeviceTables **out);
@@ -138,10 +138,10 @@ struct AfftEngine {
     uint64_t N = 0, L = 0;
     bool fused = false;     // k = 2^Dk, m <= k: interpolate then evaluate the coset [k, 2k)
     const AfftDeviceTables *dev = nullptr;
-    DevBuf enc_scale;       // systematic locator tables (general encode)
+    DevBuf enc_scale;       // systematic locator factors (general encode)
     DevBuf log_wht;         // decode locator input
     DevBuf work, work2;
-    DevBuf bits, loc_work, loc_out, dec_scale, in_rows, copy_rows, out_rows;
+    DevBuf bits, loc_work, loc_out, in_rows, copy_rows, out_rows;
     std::vector<uint32_t> h_bits;
     HostBuf h_bits_pinned;
     hipEvent_t bits_copied = nullptr;

@@ -99,14 +99,10 @@ hipError_t afft_device_tables(const AfftDeviceTables **out) {
         dscale[q] = m.exp[lg];
         dunscale[q] = m.exp[(kAfftOrder - lg) % kAfftOrder];
     }
-    auto build = [&](const std::vector<uint16_t> &c, DevBuf &tabs) -> hipError_t {
+    auto build = [&](const std::vector<uint16_t> &c, DevBuf &fac) -> hipError_t {
         hipError_t e;
-        if ((e = tabs.ensure(c.size() * 128)) != hipSuccess ||
-            (e = hipMemcpy(consts.ptr, c.data(), c.size() * 2, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = launch_afft_tables(consts.as<uint16_t>(), static_cast<uint32_t>(c.size()), tabs.as<uint16_t>(),
-                                    nullptr)) != hipSuccess)
-            return e;
-        return hipStreamSynchronize(nullptr);
+        if ((e = fac.ensure(c.size() * 2)) != hipSuccess) return e;
+        return hipMemcpy(fac.ptr, c.data(), c.size() * 2, hipMemcpyHostToDevice);
     };
     if ((err = consts.ensure(65536 * 2)) == hipSuccess &&
         (err = t->deriv_tabs.ensure(kAfftMaxLog * 128)) == hipSuccess &&
@@ -153,12 +149,8 @@ scrs_status AfftEngine::init(const scrs_profile &p, bool for_decode) {
         std::vector<uint8_t> known(N, 0);
         for (uint32_t x = 0; x < k; x++) known[x] = 1;
         const std::vector<uint16_t> c = afft_locator_scales(known, D);
-        DevBuf dc;
-        if ((err = dc.ensure(N * 2)) != hipSuccess || (err = enc_scale.ensure(N * 128)) != hipSuccess ||
-            (err = hipMemcpy(dc.ptr, c.data(), N * 2, hipMemcpyHostToDevice)) != hipSuccess ||
-            (err = launch_afft_tables(dc.as<uint16_t>(), static_cast<uint32_t>(N), enc_scale.as<uint16_t>(),
-                                      nullptr)) != hipSuccess ||
-            (err = hipStreamSynchronize(nullptr)) != hipSuccess)
+        if ((err = enc_scale.ensure(N * 2)) != hipSuccess ||
+            (err = hipMemcpy(enc_scale.ptr, c.data(), N * 2, hipMemcpyHostToDevice)) != hipSuccess)
             return fail(err);
     }
     if (for_decode) {
@@ -170,7 +162,7 @@ scrs_status AfftEngine::init(const scrs_profile &p, bool for_decode) {
             (err = hipMemcpy(log_wht.ptr, lw.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess ||
             (err = bits.ensure(words * 4)) != hipSuccess || (err = h_bits_pinned.ensure(words * 4)) != hipSuccess ||
             (err = loc_work.ensure(N * 4)) != hipSuccess || (err = loc_out.ensure(N * 2)) != hipSuccess ||
-            (err = dec_scale.ensure(N * 128)) != hipSuccess || (err = in_rows.ensure(N * 8)) != hipSuccess ||
+            (err = in_rows.ensure(N * 8)) != hipSuccess ||
             (err = copy_rows.ensure(N * 8)) != hipSuccess || (err = out_rows.ensure(N * 8)) != hipSuccess ||
             (err = work.ensure(N * L)) != hipSuccess || (err = work2.ensure(N * L)) != hipSuccess ||
             (err = hipEventCreateWithFlags(&bits_copied, hipEventDisableTiming)) != hipSuccess)
@@ -429,11 +421,8 @@ scrs_status AfftEngine::decode(const uint16_t *indices, const uint8_t *const *pt
     la.work = loc_work.as<uint32_t>();
     la.out = loc_out.as<uint16_t>();
     la.D = D;
-    if ((err = launch_afft_locator(la, s)) != hipSuccess ||
-        (err = launch_afft_tables(loc_out.as<uint16_t>(), static_cast<uint32_t>(N), dec_scale.as<uint16_t>(), s)) !=
-            hipSuccess)
-        return st(SCRS_ERR_DEVICE, err);
-    const uint16_t *sc = dec_scale.as<uint16_t>();
+    if ((err = launch_afft_locator(la, s)) != hipSuccess) return st(SCRS_ERR_DEVICE, err);
+    const uint16_t *sc = loc_out.as<uint16_t>();
     uint8_t *w1 = work.as<uint8_t>(), *w2 = work2.as<uint8_t>();
     auto in_work = [&](AfftPassArgs &a, const uint8_t *w) {
         a.in_base = w;

@@ -373,7 +373,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     constexpr bool kAbs = W && GL != 0 && FX != kFxDynamic;
     constexpr uint32_t kGc = GL ? 1u << GL : 0u;
     constexpr uint32_t kFwdByte = (FX & kFxInv) ? level_off_bytes(kGc, GL, MX, kTN) : 0u;
-    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+    const uint32_t tid = threadIdx.x, nthr = (G / R) * 64u;  // the launch's block size
 
     // Row and derivative table staging in 16-byte chunks, kStageIn chunks
     // per thread in flight at once, so that one wait covers them all: chunk
@@ -393,22 +393,25 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             for (uint32_t q = 0; q < kStageIn; q++) d[min(c0 + q * nthr + tid, chunks - 1u)] = val[q];
         }
     };
-    // The node tables of this group, built in LDS from the nodes' factors.
-    // Level beta (pass bit b_lo+beta) uses G >> (beta+1) consecutive nodes,
-    // all levels as one block per direction; table n of the block (in the
-    // pass's numbering, level by level) is of level
-    // g - 1 - floor(log2(G - 1 - n)).  A table holds the products of its
-    // factor c with every value of each address field, and c*v is the XOR
-    // of c*2^b over the bits b of v.  A work item is a 32-entry quarter of
-    // a wide table (the halves of its 6-bit field, its two 5-bit fields) or
-    // half of a nibble table (two nibble fields).  The quarter is the outer
-    // index, so that a wave's items are of one kind when a block has 64
-    // tables or more.
-    // A thread has at most two items (4 * 2G items over G / R * 64 >= 4G
-    // threads), items tid and tid + nthr.  Their factors load before the
+    // The node tables of this group and its row-scale tables, built in LDS
+    // from their factors.  Node tables: level beta (pass bit b_lo+beta)
+    // uses G >> (beta+1) consecutive nodes, all levels as one block per
+    // direction; table n of the block (in the pass's numbering, level by
+    // level) is of level g - 1 - floor(log2(G - 1 - n)).  Row tables: one
+    // per row of the group and scale present (input, output, derivative).
+    // A table holds the products of its factor c with every value of each
+    // address field, and c*v is the XOR of c*2^b over the bits b of v.
+    //
+    // A work item is a 32-entry quarter of a wide table (the halves of its
+    // 6-bit field, its two 5-bit fields) or half of a nibble table (two
+    // nibble fields).  Node items: 4 per table slot and direction, the
+    // quarter outermost so that a wave's items are of one kind when a block
+    // has 64 tables or more; a thread has at most two (8G items over
+    // G / R * 64 >= 4G threads), items tid and tid + nthr.  Row items: 2 per
+    // row, at most one per thread and scale.  The factors load before the
     // first tile's rows, so that the tables build while the rows arrive.
-    const uint32_t lp = g + ((inv && fwd) ? 1u : 0u), per = 1u << lp;  // items per quarter
-    // Node of item `item`, or ~0u for no table part.
+    const uint32_t lp = g + ((inv && fwd) ? 1u : 0u), per = 1u << lp;  // node items per quarter
+    // Node of node item `item`, or ~0u for no table part.
     auto item_node = [&](uint32_t item) -> uint32_t {
         const uint32_t q = item >> lp, rest = item & (per - 1u), n = rest & (G - 1u);
         if (!(inv || fwd) || item >= 4u * per || n == G - 1u) return ~0u;
@@ -418,77 +421,89 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
         const uint32_t pbase = ((!inv || rest >= G) ? a.fwd_base : a.inv_base) + base;
         return (1u << (15u - B)) + (pbase >> (B + 1)) + (n - first);
     };
-    uint32_t fac[2];
+    // Row of the group of row item tid: 2 items per row, the half outermost.
+    const uint32_t r_item = base | ((tid & (G - 1u)) << b_lo);
+    const bool row_item = tid < 2u * G;
+    uint32_t fac[2], fac_r[3];
     auto load_factors = [&]() {
+        // Every load unconditional, at a clamped address.
+        if (inv || fwd) {
 #pragma unroll
-        for (uint32_t k = 0; k < 2; k++) {
-            const uint32_t node = item_node(tid + k * nthr);
-            fac[k] = a.node_fac[node == ~0u ? 0u : node];
-        }
-    };
-    auto build_nodes = [&]() {
-#pragma unroll
-        for (uint32_t k = 0; k < 2; k++) {
-            const uint32_t item = tid + k * nthr;
-            if (item_node(item) == ~0u) continue;
-            const uint32_t q = item >> lp, rest = item & (per - 1u), n = rest & (G - 1u);
-            const uint32_t beta = g - 1u - (31u - static_cast<uint32_t>(__builtin_clz(G - 1u - n)));
-            const bool wide = tab_n(beta) == kTabW, use_fwd = !inv || rest >= G;
-            const uint32_t first = G - (G >> beta);
-            uint32_t cb2[16];  // c * 2^b
-            cb2[0] = fac[k];
-#pragma unroll
-            for (int b = 1; b < 8; b++) cb2[b] = xtime16(cb2[b - 1]);
-            cb2[8] = times_u(fac[k]);
-#pragma unroll
-            for (int b = 9; b < 16; b++) cb2[b] = xtime16(cb2[b - 1]);
-            // Entry h*16 + m of the item (m < 16) is f[h] + sum over the bits
-            // b of m of x[h][b].
-            uint32_t x[2][4], f[2];
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                if (wide) {
-                    x[0][b] = x[1][b] = q < 2u ? cb2[b] : (q == 2u ? cb2[6 + b] : cb2[11 + b]);
-                } else {
-                    x[0][b] = q == 0u ? cb2[b] : cb2[8 + b];
-                    x[1][b] = q == 0u ? cb2[4 + b] : cb2[12 + b];
-                }
-            }
-            f[0] = wide && q == 1u ? cb2[5] : 0u;
-            f[1] = wide ? f[0] ^ (q < 2u ? cb2[4] : (q == 2u ? cb2[10] : cb2[15])) : 0u;
-            uint4 *d = reinterpret_cast<uint4 *>((use_fwd ? fwd_t : inv_t) + level_off(beta) + (n - first) * tab_n(beta) +
-                                                 32u * q);
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                uint32_t e[8];  // entries 2i, packed with entry 2i + 1
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    uint32_t v = f[h];
-#pragma unroll
-                    for (int b = 1; b < 4; b++)
-                        if ((i >> (b - 1)) & 1) v ^= x[h][b];
-                    e[i] = v | ((v ^ x[h][0]) << 16);
-                }
-                d[2 * h] = make_uint4(e[0], e[1], e[2], e[3]);
-                d[2 * h + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+            for (uint32_t k = 0; k < 2; k++) {
+                const uint32_t node = item_node(tid + k * nthr);
+                fac[k] = a.node_fac[node == ~0u ? 0u : node];
             }
         }
+        const uint32_t r = row_item ? r_item : base;
+        if (a.scale_in) fac_r[0] = a.scale_in[r];
+        if (a.scale_out) fac_r[1] = a.scale_out[r];
+        if (a.scale_der) fac_r[2] = a.scale_der[r];
     };
-    // Per-row tables of the group's rows (kTab entries, 8 chunks, per row).
-    auto stage_rows = [&](uint16_t *dst, const uint16_t *src) {
-        stage_chunks(dst, G * (kTab / 8u), [&](uint32_t c) {
-            const uint32_t r = base | ((c / (kTab / 8u)) << b_lo);
-            return reinterpret_cast<const uint4 *>(src + r * kTab) + c % (kTab / 8u);
-        });
+    // The item of factor c: quarter (wide) or half (nibble) q, first entry
+    // at dst.
+    auto build_item = [&](uint32_t c, uint32_t q, bool wide, uint16_t *dst) {
+        uint32_t cb2[16];  // c * 2^b
+        cb2[0] = c;
+#pragma unroll
+        for (int b = 1; b < 8; b++) cb2[b] = xtime16(cb2[b - 1]);
+        cb2[8] = times_u(c);
+#pragma unroll
+        for (int b = 9; b < 16; b++) cb2[b] = xtime16(cb2[b - 1]);
+        // Entry h*16 + m of the item (m < 16) is f[h] + sum over the bits b
+        // of m of x[h][b].
+        uint32_t x[2][4], f[2];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            if (wide) {
+                x[0][b] = x[1][b] = q < 2u ? cb2[b] : (q == 2u ? cb2[6 + b] : cb2[11 + b]);
+            } else {
+                x[0][b] = q == 0u ? cb2[b] : cb2[8 + b];
+                x[1][b] = q == 0u ? cb2[4 + b] : cb2[12 + b];
+            }
+        }
+        f[0] = wide && q == 1u ? cb2[5] : 0u;
+        f[1] = wide ? f[0] ^ (q < 2u ? cb2[4] : (q == 2u ? cb2[10] : cb2[15])) : 0u;
+        uint4 *d = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t e[8];  // entries 2i, packed with entry 2i + 1
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                uint32_t v = f[h];
+#pragma unroll
+                for (int b = 1; b < 4; b++)
+                    if ((i >> (b - 1)) & 1) v ^= x[h][b];
+                e[i] = v | ((v ^ x[h][0]) << 16);
+            }
+            d[2 * h] = make_uint4(e[0], e[1], e[2], e[3]);
+            d[2 * h + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+        }
+    };
+    auto build_tables = [&]() {
+        if (inv || fwd) {
+#pragma unroll
+            for (uint32_t k = 0; k < 2; k++) {
+                const uint32_t item = tid + k * nthr;
+                if (item_node(item) == ~0u) continue;
+                const uint32_t q = item >> lp, rest = item & (per - 1u), n = rest & (G - 1u);
+                const uint32_t beta = g - 1u - (31u - static_cast<uint32_t>(__builtin_clz(G - 1u - n)));
+                const uint32_t first = G - (G >> beta);
+                build_item(fac[k], q, tab_n(beta) == kTabW,
+                           ((!inv || rest >= G) ? fwd_t : inv_t) + level_off(beta) + (n - first) * tab_n(beta) + 32u * q);
+            }
+        }
+        if (row_item) {
+            const uint32_t q = tid >> g, off = (tid & (G - 1u)) * kTab + 32u * q;
+            if (a.scale_in) build_item(fac_r[0], q, false, sin_t + off);
+            if (a.scale_out) build_item(fac_r[1], q, false, sout_t + off);
+            if (a.scale_der) build_item(fac_r[2], q, false, sder_t + off);
+        }
     };
     auto stage_tables = [&]() {
-        if (inv || fwd) build_nodes();
+        build_tables();
         if (der && !unit) stage_chunks(der_t, 16u * (kTab / 8u), [&](uint32_t c) {
             return reinterpret_cast<const uint4 *>(a.deriv_tabs) + c;
         });
-        if (a.scale_in) stage_rows(sin_t, a.scale_in);
-        if (a.scale_out) stage_rows(sout_t, a.scale_out);
-        if (a.scale_der) stage_rows(sder_t, a.scale_der);
         __syncthreads();
     };
 
