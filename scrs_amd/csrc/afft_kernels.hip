@@ -442,27 +442,46 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // The item of factor c: quarter (wide) or half (nibble) q, first entry
     // at dst.
     auto build_item = [&](uint32_t c, uint32_t q, bool wide, uint16_t *dst) {
-        uint32_t cb2[16];  // c * 2^b
-        cb2[0] = c;
-#pragma unroll
-        for (int b = 1; b < 8; b++) cb2[b] = xtime16(cb2[b - 1]);
-        cb2[8] = times_u(c);
-#pragma unroll
-        for (int b = 9; b < 16; b++) cb2[b] = xtime16(cb2[b - 1]);
         // Entry h*16 + m of the item (m < 16) is f[h] + sum over the bits b
-        // of m of x[h][b].
+        // of m of x[h][b].  Each kind of item forms only the c*2^b it uses
+        // (a wave's items are of one kind when a block has 64 tables or more).
         uint32_t x[2][4], f[2];
+        auto run = [&](uint32_t v, uint32_t(&o)[4]) {  // o[b] = v * 2^b
+            o[0] = v;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            if (wide) {
-                x[0][b] = x[1][b] = q < 2u ? cb2[b] : (q == 2u ? cb2[6 + b] : cb2[11 + b]);
-            } else {
-                x[0][b] = q == 0u ? cb2[b] : cb2[8 + b];
-                x[1][b] = q == 0u ? cb2[4 + b] : cb2[12 + b];
-            }
+            for (int b = 1; b < 4; b++) o[b] = xtime16(o[b - 1]);
+        };
+        if (wide && q < 2u) {  // bits 0..5
+            run(c, x[0]);
+            const uint32_t c16 = xtime16(x[0][3]), c32 = xtime16(c16);
+            f[0] = q == 1u ? c32 : 0u;
+            f[1] = f[0] ^ c16;
+            run(c, x[1]);
+        } else if (wide && q == 2u) {  // bits 6..10: c x^6, c x^7, c u, c u x, c u x^2
+            uint32_t c64 = c;
+#pragma unroll
+            for (int b = 0; b < 6; b++) c64 = xtime16(c64);
+            const uint32_t cu = times_u(c), cu2 = xtime16(cu);
+            x[0][0] = x[1][0] = c64;
+            x[0][1] = x[1][1] = xtime16(c64);
+            x[0][2] = x[1][2] = cu;
+            x[0][3] = x[1][3] = cu2;
+            f[0] = 0u;
+            f[1] = xtime16(cu2);
+        } else if (wide) {  // bits 11..15: c u x^3 .. c u x^7
+            uint32_t cu8 = times_u(c);
+#pragma unroll
+            for (int b = 0; b < 3; b++) cu8 = xtime16(cu8);
+            run(cu8, x[0]);
+            f[0] = 0u;
+            f[1] = xtime16(x[0][3]);
+            run(cu8, x[1]);
+        } else {  // nibble fields 2q and 2q + 1
+            const uint32_t v = q == 0u ? c : times_u(c);
+            run(v, x[0]);
+            run(xtime16(x[0][3]), x[1]);
+            f[0] = f[1] = 0u;
         }
-        f[0] = wide && q == 1u ? cb2[5] : 0u;
-        f[1] = wide ? f[0] ^ (q < 2u ? cb2[4] : (q == 2u ? cb2[10] : cb2[15])) : 0u;
         uint4 *d = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
         for (int h = 0; h < 2; h++) {
