@@ -766,8 +766,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             }
         }
     };
-    auto stage_lo = [&](uint32_t k) { return k * RB; };
-    auto stage_t = [&](uint32_t k) { return min(RB, g - k * RB); };
+    // With the level count fixed, the first stage takes the remainder of g
+    // over RB bits, so that the top stage is a full one: a 7-level pass runs
+    // 3 levels, then 4 at compile-time table addresses (kAbs).  Otherwise the
+    // last stage takes it (the register budget of the general kernels).
+    constexpr bool kShortFirst = GL != 0 && GL % RB != 0;
+    constexpr uint32_t kT0 = kShortFirst ? GL % RB : RB;
+    auto stage_lo = [&](uint32_t k) { return kShortFirst ? (k == 0u ? 0u : kT0 + (k - 1u) * RB) : k * RB; };
+    auto stage_t = [&](uint32_t k) { return kShortFirst ? (k == 0u ? kT0 : RB) : min(RB, g - k * RB); };
 
     const uint32_t tiles = static_cast<uint32_t>((L + kTileDwords * 4 - 1) / (kTileDwords * 4));
     const uint32_t t_first = blockIdx.x * tpb, t_end = min(tiles, t_first + tpb);
