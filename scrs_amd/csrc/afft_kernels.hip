@@ -235,6 +235,48 @@ __device__ __forceinline__ void gmul2_issue_abs(Parts<true> &p, uint32_t off, ui
     p.r[4] = lds_u16(and_hi(0x3Eu, w5) + off + 128u);
     p.r[5] = lds_u16(and_hi(0x3Eu, w10) + off + 192u);
 }
+// Below the top stage a level's tables sit at a wave-uniform base.  A wide
+// table's LDS byte address is a multiple of 256 and its entry offsets (with
+// the field's table part in the instruction's offset field) are below 256,
+// so an address register that holds the base keeps it in bits 8 and up,
+// and each product rewrites only its low byte: an AND under SDWA with the
+// other bytes preserved, one instruction per address as with compile-time
+// addresses.  a.byte0 = w & m (lo) or w.hi & m (hi).  Pure ALU.
+__device__ __forceinline__ void set_byte0(uint32_t &a, uint32_t m, uint32_t w) {
+    asm("v_and_b32_sdwa %0, %1, %2 dst_sel:BYTE_0 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:DWORD"
+        : "+v"(a)
+        : "s"(m), "v"(w));
+}
+__device__ __forceinline__ void set_byte0_hi(uint32_t &a, uint32_t m, uint32_t w) {
+    asm("v_and_b32_sdwa %0, %1, %2 dst_sel:BYTE_0 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:WORD_1"
+        : "+v"(a)
+        : "s"(m), "v"(w));
+}
+// The six address registers of a wide product (two elements, three fields).
+struct WideAddr {
+    uint32_t a[6];
+};
+__device__ __forceinline__ void wide_addr_init(WideAddr &ad, uint32_t base) {
+#pragma unroll
+    for (int f = 0; f < 6; f++) ad.a[f] = base;
+}
+// gmul2_issue (wide format) through the address registers of a table base
+// (a multiple of 256); `off` is the table's constant offset from the base.
+__device__ __forceinline__ void gmul2_issue_base(Parts<true> &p, WideAddr &ad, uint32_t off, uint32_t w) {
+    const uint32_t w2 = w << 1, w5 = w >> 5, w10 = w >> 10;
+    set_byte0(ad.a[0], 0x7Eu, w2);
+    p.r[0] = lds_u16(ad.a[0] + off);
+    set_byte0(ad.a[1], 0x3Eu, w5);
+    p.r[1] = lds_u16(ad.a[1] + off + 128u);
+    set_byte0(ad.a[2], 0x3Eu, w10);
+    p.r[2] = lds_u16(ad.a[2] + off + 192u);
+    set_byte0_hi(ad.a[3], 0x7Eu, w2);
+    p.r[3] = lds_u16(ad.a[3] + off);
+    set_byte0_hi(ad.a[4], 0x3Eu, w5);
+    p.r[4] = lds_u16(ad.a[4] + off + 128u);
+    set_byte0_hi(ad.a[5], 0x3Eu, w10);
+    p.r[5] = lds_u16(ad.a[5] + off + 192u);
+}
 // Byte offset of level beta's first node table in a pass's block of node
 // tables over G rows (tn u16 entries per table; mixed: the first level's
 // tables in the nibble format).
@@ -371,6 +413,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
     // a fixed feature set and wide tables.  kTabsByte(inverse): the LDS byte
     // address of that direction's node tables.
     constexpr bool kAbs = W && GL != 0 && FX != kFxDynamic;
+    // Address registers for the wide levels below the top stage (the fixed
+    // kernels; the general ones keep their registers for their branches).
+    constexpr bool kBaseRegs = kAbs;
     constexpr uint32_t kGc = GL ? 1u << GL : 0u;
     constexpr uint32_t kFwdByte = (FX & kFxInv) ? level_off_bytes(kGc, GL, MX, kTN) : 0u;
     const uint32_t tid = threadIdx.x, nthr = (G / R) * 64u;  // the launch's block size
@@ -656,6 +701,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
 #pragma unroll
                 for (int q = 0; q < NU; q++) lb[q] = level_base(tabs, s_lo, T, bs, q);
             }
+            // Wide levels below the top stage: address registers (above).
+            constexpr bool BASE = !ABS && LW && kBaseRegs;
+            WideAddr ad[BASE ? NU : 1];
+            if constexpr (BASE) {
+#pragma unroll
+                for (int q = 0; q < NU; q++) wide_addr_init(ad[q], lb[q]);
+            }
             // Butterfly n of the level: registers jn and jn + 2^bs.
             auto jn = [&](int n) { return ((n >> bs) << (bs + 1)) | (n & ((1 << bs) - 1)); };
             auto off = [&](int j) { return static_cast<uint32_t>((j & ((1 << T) - 1)) >> (bs + 1)) * (kLN * 2); };
@@ -670,6 +722,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
                     if (!zero(j) && !dead(j)) {
                         if constexpr (ABS)
                             gmul2_issue_abs(pp[n % (kPipe + 1)], kLevelByte + off(j), v[j + (1 << bs)]);
+                        else if constexpr (BASE)
+                            gmul2_issue_base(pp[n % (kPipe + 1)], ad[j >> T], off(j), v[j + (1 << bs)]);
                         else
                             gmul2_issue<LW>(pp[n % (kPipe + 1)], lb[j >> T], off(j), v[j + (1 << bs)]);
                     }
