@@ -527,20 +527,36 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SCRS_AFFT_
             run(xtime16(x[0][3]), x[1]);
             f[0] = f[1] = 0u;
         }
-        uint4 *d = reinterpret_cast<uint4 *>(dst);
+        uint32_t e[16];  // entries 2i, packed with entry 2i + 1: four 16-byte chunks
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            uint32_t e[8];  // entries 2i, packed with entry 2i + 1
+        for (int h = 0; h < 2; h++)
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 uint32_t v = f[h];
 #pragma unroll
                 for (int b = 1; b < 4; b++)
                     if ((i >> (b - 1)) & 1) v ^= x[h][b];
-                e[i] = v | ((v ^ x[h][0]) << 16);
+                e[8 * h + i] = v | ((v ^ x[h][0]) << 16);
             }
-            d[2 * h] = make_uint4(e[0], e[1], e[2], e[3]);
-            d[2 * h + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+        // Neighbouring lanes build neighbouring tables, 256 or 128 bytes
+        // apart: the same banks.  Store i of a lane writes chunk i ^ (lane
+        // mod 4), so that one store spreads over four chunk positions
+        // (16-way bank conflicts instead of 64-way).
+        uint4 *d = reinterpret_cast<uint4 *>(dst);
+        const uint32_t sw = tid & 3u;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t c = i ^ sw;
+            // Chunk c by masks rather than selects, which the compiler turns
+            // into an indexed read of e[] from scratch memory.
+            const uint32_t m1 = 0u - (c & 1u), m2 = 0u - ((c >> 1) & 1u);
+            uint32_t w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t lo = e[u] ^ ((e[u] ^ e[4 + u]) & m1), hi = e[8 + u] ^ ((e[8 + u] ^ e[12 + u]) & m1);
+                w[u] = lo ^ ((lo ^ hi) & m2);
+            }
+            d[c] = make_uint4(w[0], w[1], w[2], w[3]);
         }
     };
     auto build_tables = [&]() {
