@@ -62,7 +62,7 @@ BitParts plan_bits(uint32_t D, uint32_t cap) {
     return parts;
 }
 
-// Largest pass that still fits every table a single-pass decode stages.
+// Largest pass whose LDS still holds every table a single-pass decode builds.
 constexpr uint32_t kSinglePassLog = 7;
 #ifndef SCRS_AFFT_PASS_LOG
 #define SCRS_AFFT_PASS_LOG 8
